@@ -1,0 +1,96 @@
+// gp_internal.h — device data layout of one topology bucket and the kernel
+// launch interface shared by gp_kernels.hip and the host runtime.
+//
+// A bucket holds B instances that share a topology signature (body count,
+// parents, DOF class per body, contact points per body, half-space count).
+// Every per-instance array is structure-of-arrays with a lane pitch Bp
+// (B rounded up to 64): element (field f, item i, lane l) lives at
+// [(f * n_items + i) * Bp + l], so a wavefront reads 64 consecutive values.
+//
+// Body parameters are stored in each body's *aligned* frame: a 1-DOF joint's
+// axis is the local z axis (the host rotates the reference's body frame by a
+// fixed rotation A_i with A_i e_z = axis). The motion subspace is then e_z
+// (revolute: angular z, prismatic: linear z), U = column 2 or 5 of I^A and
+// D = its diagonal entry.
+#pragma once
+
+#include <cstdint>
+
+namespace gp {
+
+enum Kind { K_FIXED = 0, K_REV = 1, K_PRI = 2, K_FLOAT = 3 };
+
+// body parameter fields
+enum BodyField {
+  BF_RT = 0,   // 9: tree rotation, child-aligned -> parent-aligned (row-major)
+  BF_PT = 9,   // 3: tree translation, parent-aligned coords
+  BF_J = 12,   // 6: moment about origin xx,yy,zz,xy,xz,yz
+  BF_C = 18,   // 3: m * com
+  BF_M = 21,   // 1: mass
+  BF_SK = 22,  // 1: prismatic spring k
+  BF_SL = 23,  // 1: spring rest length
+  NBF = 24
+};
+enum ContactField { CF_P = 0, CF_K = 3, NCF = 4 };
+enum HalfspaceField { HF_POINT = 0, HF_NORMAL = 3, HF_ALPHA = 6, HF_MU = 7, NHF = 8 };
+
+// per-lane scratch fields of the runtime-topology (plain) kernel, per body
+enum PlainField {
+  PF_E = 0,     // 9
+  PF_R = 9,     // 3
+  PF_VEL = 12,  // 6
+  PF_CB = 18,   // 6
+  PF_PA = 24,   // 6
+  PF_IA = 30,   // 21: A(6) B(9) C(6)
+  PF_U = 51,    // 6
+  PF_D = 57,    // 1
+  PF_UU = 58,   // 6 (1-DOF: u in [0]; floating: u[6])
+  PF_ACC = 64,  // 6
+  PF_RW = 70,   // 9 world rotation
+  PF_TW = 79,   // 3 world translation
+  NPF = 82
+};
+
+constexpr int kMaxBodies = 64;
+constexpr double kInsideMargin = 1e-8;  // halfspace.rs:42
+constexpr double kHalfspaceK = 50e3;    // contact.rs:328
+constexpr double kSlipTol = 1e-3;       // contact.rs:287
+
+struct BucketArgs {
+  int B;      // instances
+  int Bp;     // lane pitch
+  int nb, nq, nv, nc, nh;
+  int lanes;  // lanes per instance (FAST kernels)
+  double g;
+  // topology (device, uniform over the bucket)
+  const int* parent;  // [nb]
+  const int* qoff;    // [nb]
+  const int* voff;    // [nb]
+  const int* cstart;  // [nb + 1] contacts sorted by body
+  // parameters
+  const int* kind;     // [nb][Bp]
+  const double* bp;    // [NBF][nb][Bp]
+  const float* bpf;    // same, FP32
+  const double* cp;    // [NCF][nc][Bp]
+  const float* cpf;
+  const double* hp;    // [NHF][nh][Bp]
+  const float* hpf;
+  // state
+  double* q;           // [nq][Bp]
+  double* v;           // [nv][Bp]
+  const double* tau;   // [nv][Bp]
+  double* vdot;        // [nv][Bp] acceleration of the last step
+  int* fault;          // [Bp]
+  const int* active;   // [Bp] or null: per-lane remaining steps (serving)
+  double* scratch;     // plain kernel: [NPF][nb][Bp] + RK stages
+};
+
+// Host-side launchers (gp_kernels.hip). Each returns a hipError_t as int.
+int launch_plain_step(const BucketArgs& a, double dt, int integrator, int dynamics_only, void* stream);
+// Fast kernels: topo_id from fast_topology_id(); precision GP_PREC_*.
+int fast_topology_id(int nb, const int* parent, const int* dofcls, int nc);
+int launch_fast_step(int topo_id, int precision, const BucketArgs& a, double dt, int n_steps, int block,
+                     void* stream);
+size_t plain_scratch_doubles(int nb, int nq, int nv, int Bp);
+
+}  // namespace gp

@@ -1,0 +1,707 @@
+// gp_runtime.cpp — host side of the C ABI (include/gorilla_amd.h).
+//
+// Model validation and the reference's flattening rules, bucketing of
+// instances by topology, structure-of-arrays packing into device memory,
+// step dispatch to the HIP kernels (plain / fast), hipGraph capture of
+// multi-launch rollouts, the reference's simulate loop, fault flags and
+// HIP-event timing. No C++ exception leaves this file: every entry point
+// returns a status code and records a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/gorilla_amd.h"
+#include "gp_internal.h"
+#include "gp_runtime.h"
+
+namespace gp {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(GP_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ host math
+namespace {
+
+struct M {
+  double a[9];
+};
+M mmul(const M& A, const M& B) {
+  M C;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C.a[3 * i + j] = A.a[3 * i] * B.a[j] + A.a[3 * i + 1] * B.a[3 + j] + A.a[3 * i + 2] * B.a[6 + j];
+  return C;
+}
+M mT(const M& A) { return M{{A.a[0], A.a[3], A.a[6], A.a[1], A.a[4], A.a[7], A.a[2], A.a[5], A.a[8]}}; }
+void mv(const M& A, const double* v, double* o) {
+  double t[3];
+  for (int i = 0; i < 3; ++i) t[i] = A.a[3 * i] * v[0] + A.a[3 * i + 1] * v[1] + A.a[3 * i + 2] * v[2];
+  o[0] = t[0]; o[1] = t[1]; o[2] = t[2];
+}
+M ident() { return M{{1, 0, 0, 0, 1, 0, 0, 0, 1}}; }
+M quat_to_m(const double* q) {  // (x, y, z, w), nalgebra to_rotation_matrix
+  double i = q[0], j = q[1], k = q[2], w = q[3];
+  double ww = w * w, ii = i * i, jj = j * j, kk = k * k;
+  double ij = 2 * i * j, wk = 2 * w * k, wj = 2 * w * j, ik = 2 * i * k, jk = 2 * j * k, wi = 2 * w * i;
+  return M{{ww + ii - jj - kk, ij - wk, wj + ik, wk + ij, ww - ii + jj - kk, jk - wi, ik - wj, wi + jk, ww - ii - jj + kk}};
+}
+// rotation A with A e_z = a: exact signed permutation for +-e_x,y,z
+M align_z(const double* a) {
+  int nz = (a[0] != 0) + (a[1] != 0) + (a[2] != 0);
+  auto cols = [](const double* b1, const double* b2, const double* b3) {
+    return M{{b1[0], b2[0], b3[0], b1[1], b2[1], b3[1], b1[2], b2[2], b3[2]}};
+  };
+  if (nz == 1) {
+    if (a[2] != 0) {
+      double s = a[2] > 0 ? 1 : -1;
+      double b1[3] = {1, 0, 0}, b2[3] = {0, s, 0}, b3[3] = {0, 0, s};
+      return cols(b1, b2, b3);
+    } else if (a[0] != 0) {
+      double s = a[0] > 0 ? 1 : -1;
+      double b1[3] = {0, s, 0}, b2[3] = {0, 0, 1}, b3[3] = {s, 0, 0};
+      return cols(b1, b2, b3);
+    } else {
+      double s = a[1] > 0 ? 1 : -1;
+      double b1[3] = {0, 0, s}, b2[3] = {1, 0, 0}, b3[3] = {0, s, 0};
+      return cols(b1, b2, b3);
+    }
+  }
+  double t[3] = {std::fabs(a[0]) < 0.9 ? 1.0 : 0.0, std::fabs(a[0]) < 0.9 ? 0.0 : 1.0, 0.0};
+  double d = t[0] * a[0] + t[1] * a[1] + t[2] * a[2];
+  double b1[3] = {t[0] - d * a[0], t[1] - d * a[1], t[2] - d * a[2]};
+  double n = std::sqrt(b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2]);
+  for (double& x : b1) x /= n;
+  double b2[3] = {a[1] * b1[2] - a[2] * b1[1], a[2] * b1[0] - a[0] * b1[2], a[0] * b1[1] - a[1] * b1[0]};
+  return cols(b1, b2, a);
+}
+
+uint64_t fnv(uint64_t h, int64_t x) {
+  for (int k = 0; k < 8; ++k) {
+    h ^= (uint64_t)((x >> (8 * k)) & 0xff);
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+bool finite_all(const double* p, int n) {
+  for (int k = 0; k < n; ++k)
+    if (!std::isfinite(p[k])) return false;
+  return true;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ model
+int model_create(const gp_model_desc* d, gp_model** out) {
+  if (!d || !out) return fail(GP_ERR_INVALID, "null argument");
+  if (d->n_bodies < 1 || d->n_bodies > kMaxBodies)
+    return fail(GP_ERR_INVALID, "n_bodies %d outside [1, %d]", d->n_bodies, kMaxBodies);
+  if (!d->bodies) return fail(GP_ERR_INVALID, "bodies is null");
+  if (d->n_contacts < 0 || (d->n_contacts > 0 && !d->contacts)) return fail(GP_ERR_INVALID, "bad contacts");
+  if (d->n_halfspaces < 0 || (d->n_halfspaces > 0 && !d->halfspaces)) return fail(GP_ERR_INVALID, "bad half-spaces");
+  if (!std::isfinite(d->gravity)) return fail(GP_ERR_INVALID, "gravity not finite");
+  std::unique_ptr<gp_model> m(new (std::nothrow) gp_model());
+  if (!m) return fail(GP_ERR_NOMEM, "out of host memory");
+  const int n = d->n_bodies;
+  m->n = n;
+  m->g = d->gravity;
+  m->parent.resize(n); m->dofcls.resize(n); m->kind.resize(n); m->qoff.resize(n); m->voff.resize(n);
+  m->bp.assign((size_t)n * NBF, 0.0);
+  std::vector<M> Al(n);
+  for (int i = 0; i < n; ++i) {
+    const gp_body_desc& b = d->bodies[i];
+    // mechanism.rs:91-115 panics on a bad topology; here it is an error code
+    if (b.parent < -1 || b.parent >= i)
+      return fail(GP_ERR_TOPOLOGY, "body %d: parent %d must be -1 or a lower index", i, b.parent);
+    if (b.jtype < GP_JOINT_FIXED || b.jtype > GP_JOINT_FLOATING)
+      return fail(GP_ERR_TOPOLOGY, "body %d: unknown joint type %d", i, b.jtype);
+    if (!finite_all(b.axis, 3) || !finite_all(b.x_tree_quat, 4) || !finite_all(b.x_tree_trans, 3) ||
+        !finite_all(b.moment, 6) || !finite_all(b.mc, 3) || !std::isfinite(b.mass) || !std::isfinite(b.spring_k) ||
+        !std::isfinite(b.spring_l))
+      return fail(GP_ERR_INVALID, "body %d: non-finite parameter", i);
+    if (b.mass < 0) return fail(GP_ERR_INVALID, "body %d: negative mass", i);
+    double qn = std::sqrt(b.x_tree_quat[0] * b.x_tree_quat[0] + b.x_tree_quat[1] * b.x_tree_quat[1] +
+                          b.x_tree_quat[2] * b.x_tree_quat[2] + b.x_tree_quat[3] * b.x_tree_quat[3]);
+    if (std::fabs(qn - 1.0) > 1e-6) return fail(GP_ERR_INVALID, "body %d: x_tree_quat is not a unit quaternion", i);
+    m->parent[i] = b.parent;
+    int jt = b.jtype;
+    m->kind[i] = jt == GP_JOINT_REVOLUTE ? K_REV : jt == GP_JOINT_PRISMATIC ? K_PRI : jt == GP_JOINT_FLOATING ? K_FLOAT : K_FIXED;
+    m->dofcls[i] = jt == GP_JOINT_FLOATING ? 6 : jt == GP_JOINT_FIXED ? 0 : 1;
+    m->qoff[i] = m->nq;
+    m->voff[i] = m->nv;
+    m->nq += jt == GP_JOINT_FLOATING ? 7 : m->dofcls[i];
+    m->nv += m->dofcls[i];
+    if (m->dofcls[i] == 1) {
+      double an = std::sqrt(b.axis[0] * b.axis[0] + b.axis[1] * b.axis[1] + b.axis[2] * b.axis[2]);
+      if (std::fabs(an - 1.0) > 1e-6) return fail(GP_ERR_INVALID, "body %d: joint axis is not a unit vector", i);
+      double a[3] = {b.axis[0] / an, b.axis[1] / an, b.axis[2] / an};
+      if (an == 1.0) { a[0] = b.axis[0]; a[1] = b.axis[1]; a[2] = b.axis[2]; }
+      Al[i] = align_z(a);
+    } else {
+      Al[i] = ident();
+    }
+    double qq[4] = {b.x_tree_quat[0] / qn, b.x_tree_quat[1] / qn, b.x_tree_quat[2] / qn, b.x_tree_quat[3] / qn};
+    if (qn == 1.0) std::memcpy(qq, b.x_tree_quat, sizeof qq);
+    const M Ap = b.parent < 0 ? ident() : Al[b.parent];
+    const M Rt = mmul(mmul(mT(Ap), quat_to_m(qq)), Al[i]);
+    double pt[3];
+    mv(mT(Ap), b.x_tree_trans, pt);
+    M J{{b.moment[0], b.moment[3], b.moment[4], b.moment[3], b.moment[1], b.moment[5], b.moment[4], b.moment[5], b.moment[2]}};
+    M Ja = mmul(mmul(mT(Al[i]), J), Al[i]);
+    double c[3];
+    mv(mT(Al[i]), b.mc, c);
+    double* P = &m->bp[(size_t)i * NBF];
+    for (int k = 0; k < 9; ++k) P[BF_RT + k] = Rt.a[k];
+    for (int k = 0; k < 3; ++k) P[BF_PT + k] = pt[k];
+    P[BF_J + 0] = Ja.a[0]; P[BF_J + 1] = Ja.a[4]; P[BF_J + 2] = Ja.a[8];
+    P[BF_J + 3] = Ja.a[1]; P[BF_J + 4] = Ja.a[2]; P[BF_J + 5] = Ja.a[5];
+    for (int k = 0; k < 3; ++k) P[BF_C + k] = c[k];
+    P[BF_M] = b.mass;
+    P[BF_SK] = jt == GP_JOINT_PRISMATIC ? b.spring_k : 0.0;
+    P[BF_SL] = b.spring_l;
+  }
+  // contacts, stably sorted by body (contact.rs:103-128 visits bodies in order)
+  std::vector<int> order(d->n_contacts);
+  for (int c = 0; c < d->n_contacts; ++c) {
+    const gp_contact_desc& cd = d->contacts[c];
+    if (cd.body < 0 || cd.body >= n) return fail(GP_ERR_INVALID, "contact %d: body %d out of range", c, cd.body);
+    if (!finite_all(cd.p, 3) || !std::isfinite(cd.k) || cd.k <= 0) return fail(GP_ERR_INVALID, "contact %d: bad parameters", c);
+    order[c] = c;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d->contacts[x].body < d->contacts[y].body; });
+  m->nc = d->n_contacts;
+  m->cstart.assign(n + 1, 0);
+  m->cp.assign((size_t)m->nc * NCF, 0.0);
+  m->contact_order = order;
+  for (int s = 0; s < m->nc; ++s) {
+    const gp_contact_desc& cd = d->contacts[order[s]];
+    m->cstart[cd.body + 1]++;
+    double p[3];
+    mv(mT(Al[cd.body]), cd.p, p);
+    for (int k = 0; k < 3; ++k) m->cp[(size_t)s * NCF + CF_P + k] = p[k];
+    m->cp[(size_t)s * NCF + CF_K] = cd.k;
+  }
+  for (int i = 0; i < n; ++i) m->cstart[i + 1] += m->cstart[i];
+  m->nh = d->n_halfspaces;
+  m->hp.assign((size_t)m->nh * NHF, 0.0);
+  for (int h = 0; h < m->nh; ++h) {
+    const gp_halfspace_desc& hd = d->halfspaces[h];
+    if (!finite_all(hd.point, 3) || !finite_all(hd.normal, 3) || !std::isfinite(hd.alpha) || !std::isfinite(hd.mu))
+      return fail(GP_ERR_INVALID, "half-space %d: non-finite parameter", h);
+    double nn = std::sqrt(hd.normal[0] * hd.normal[0] + hd.normal[1] * hd.normal[1] + hd.normal[2] * hd.normal[2]);
+    if (std::fabs(nn - 1.0) > 1e-6) return fail(GP_ERR_INVALID, "half-space %d: normal is not a unit vector", h);
+    double* H = &m->hp[(size_t)h * NHF];
+    for (int k = 0; k < 3; ++k) { H[HF_POINT + k] = hd.point[k]; H[HF_NORMAL + k] = hd.normal[k]; }
+    H[HF_ALPHA] = hd.alpha;
+    H[HF_MU] = hd.mu;
+  }
+  // topology signature: bodies, parents, DOF class, contacts per body, half-spaces
+  uint64_t h = 1469598103934665603ull;
+  h = fnv(h, n);
+  for (int i = 0; i < n; ++i) {
+    h = fnv(h, m->parent[i]);
+    h = fnv(h, m->dofcls[i]);
+    h = fnv(h, m->cstart[i + 1] - m->cstart[i]);
+  }
+  h = fnv(h, m->nh);
+  m->sig = h;
+  m->fast_topo = fast_topology_id(n, m->parent.data(), m->dofcls.data(), m->nc);
+  *out = m.release();
+  return GP_OK;
+}
+
+// ------------------------------------------------------------------ buckets
+Bucket::~Bucket() { release(); }
+
+void Bucket::release() {
+  void* ptrs[] = {d_parent, d_qoff, d_voff, d_cstart, d_kind, d_bp, d_bpf, d_cp, d_cpf, d_hp, d_hpf,
+                  d_q, d_v, d_tau, d_tau_tmp, d_vdot, d_fault, d_scratch, d_active};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  d_parent = d_qoff = d_voff = d_cstart = d_kind = d_fault = d_active = nullptr;
+  d_bp = d_cp = d_hp = d_q = d_v = d_tau = d_tau_tmp = d_vdot = d_scratch = nullptr;
+  d_bpf = d_cpf = d_hpf = nullptr;
+}
+
+template <class T>
+static int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  return GP_OK;
+}
+template <class T>
+static int upload(T* d, const std::vector<T>& h) {
+  if (h.empty()) return GP_OK;
+  hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "hipMemcpy H2D: %s", hipGetErrorString(e));
+  return GP_OK;
+}
+
+// Allocate and fill one bucket from the models of its lanes.
+int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models) {
+  const gp_model* m0 = lane_models[0];
+  b.nb = m0->n; b.nq = m0->nq; b.nv = m0->nv; b.nc = m0->nc; b.nh = m0->nh; b.g = m0->g;
+  b.parent = m0->parent; b.qoff = m0->qoff; b.voff = m0->voff; b.dofcls = m0->dofcls; b.cstart = m0->cstart;
+  b.fast_topo = m0->fast_topo;
+  b.B = (int)lane_models.size();
+  b.Bp = (b.B + 63) / 64 * 64;
+  const size_t Bp = b.Bp;
+  std::vector<int> kind((size_t)b.nb * Bp, K_FIXED);
+  std::vector<double> bp((size_t)NBF * b.nb * Bp, 0.0), cp((size_t)NCF * b.nc * Bp, 0.0), hp((size_t)NHF * b.nh * Bp, 0.0);
+  for (int l = 0; l < b.B; ++l) {
+    const gp_model* m = lane_models[l];
+    for (int i = 0; i < b.nb; ++i) {
+      kind[i * Bp + l] = m->kind[i];
+      for (int f = 0; f < NBF; ++f) bp[((size_t)f * b.nb + i) * Bp + l] = m->bp[(size_t)i * NBF + f];
+    }
+    for (int c = 0; c < b.nc; ++c)
+      for (int f = 0; f < NCF; ++f) cp[((size_t)f * b.nc + c) * Bp + l] = m->cp[(size_t)c * NCF + f];
+    for (int h = 0; h < b.nh; ++h)
+      for (int f = 0; f < NHF; ++f) hp[((size_t)f * b.nh + h) * Bp + l] = m->hp[(size_t)h * NHF + f];
+  }
+  std::vector<float> bpf(bp.begin(), bp.end()), cpf(cp.begin(), cp.end()), hpf(hp.begin(), hp.end());
+  int st;
+  if ((st = dalloc(&b.d_parent, b.nb)) || (st = dalloc(&b.d_qoff, b.nb)) || (st = dalloc(&b.d_voff, b.nb)) ||
+      (st = dalloc(&b.d_cstart, b.nb + 1)) || (st = dalloc(&b.d_kind, kind.size())) || (st = dalloc(&b.d_bp, bp.size())) ||
+      (st = dalloc(&b.d_bpf, bpf.size())) || (st = dalloc(&b.d_cp, cp.size())) || (st = dalloc(&b.d_cpf, cpf.size())) ||
+      (st = dalloc(&b.d_hp, hp.size())) || (st = dalloc(&b.d_hpf, hpf.size())) ||
+      (st = dalloc(&b.d_q, (size_t)std::max(1, b.nq) * Bp)) || (st = dalloc(&b.d_v, (size_t)std::max(1, b.nv) * Bp)) ||
+      (st = dalloc(&b.d_tau, (size_t)std::max(1, b.nv) * Bp)) || (st = dalloc(&b.d_tau_tmp, (size_t)std::max(1, b.nv) * Bp)) ||
+      (st = dalloc(&b.d_vdot, (size_t)std::max(1, b.nv) * Bp)) || (st = dalloc(&b.d_fault, Bp)))
+    return st;
+  if ((st = upload(b.d_parent, b.parent)) || (st = upload(b.d_qoff, b.qoff)) || (st = upload(b.d_voff, b.voff)) ||
+      (st = upload(b.d_cstart, b.cstart)) || (st = upload(b.d_kind, kind)) || (st = upload(b.d_bp, bp)) ||
+      (st = upload(b.d_bpf, bpf)) || (st = upload(b.d_cp, cp)) || (st = upload(b.d_cpf, cpf)) ||
+      (st = upload(b.d_hp, hp)) || (st = upload(b.d_hpf, hpf)))
+    return st;
+  // zero state (mechanism.rs:68-88): q = 0, floating q = identity pose, v = 0, tau = 0
+  std::vector<double> q0((size_t)std::max(1, b.nq) * Bp, 0.0);
+  for (int i = 0; i < b.nb; ++i)
+    if (b.dofcls[i] == 6)
+      for (size_t l = 0; l < Bp; ++l) q0[(size_t)(b.qoff[i] + 3) * Bp + l] = 1.0;
+  if ((st = upload(b.d_q, q0))) return st;
+  HIP_TRY(hipMemset(b.d_v, 0, sizeof(double) * std::max(1, b.nv) * Bp));
+  HIP_TRY(hipMemset(b.d_tau, 0, sizeof(double) * std::max(1, b.nv) * Bp));
+  HIP_TRY(hipMemset(b.d_vdot, 0, sizeof(double) * std::max(1, b.nv) * Bp));
+  HIP_TRY(hipMemset(b.d_fault, 0, sizeof(int) * Bp));
+  return GP_OK;
+}
+
+int bucket_ensure_scratch(Bucket& b) {
+  if (b.d_scratch) return GP_OK;
+  return dalloc(&b.d_scratch, plain_scratch_doubles(b.nb, b.nq, b.nv, b.Bp));
+}
+
+BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
+  BucketArgs a{};
+  a.B = b.B; a.Bp = b.Bp; a.nb = b.nb; a.nq = b.nq; a.nv = b.nv; a.nc = b.nc; a.nh = b.nh;
+  a.lanes = lanes; a.g = b.g;
+  a.parent = b.d_parent; a.qoff = b.d_qoff; a.voff = b.d_voff; a.cstart = b.d_cstart;
+  a.kind = b.d_kind; a.bp = b.d_bp; a.bpf = b.d_bpf; a.cp = b.d_cp; a.cpf = b.d_cpf; a.hp = b.d_hp; a.hpf = b.d_hpf;
+  a.q = b.d_q; a.v = b.d_v; a.tau = tau ? tau : b.d_tau; a.vdot = b.d_vdot; a.fault = b.d_fault;
+  a.active = b.d_active; a.scratch = b.d_scratch;
+  return a;
+}
+
+}  // namespace gp
+
+using namespace gp;
+
+// ------------------------------------------------------------------ batch
+static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps, hipStream_t s, int64_t* launches) {
+  const gp_batch_opts& o = bt->opts;
+  for (Bucket& b : bt->buckets) {
+    bool fast = o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0 && integrator == GP_SEMI_IMPLICIT_EULER;
+    if (o.kernel == GP_KERNEL_FAST && !fast)
+      return fail(GP_ERR_UNSUPPORTED, "no FAST kernel for this bucket's topology / integrator");
+    if (fast) {
+      int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
+      int lanes = b.fast_topo == 100 && o.lanes_per_mech > 0 ? o.lanes_per_mech : 1;
+      BucketArgs a = bucket_args(b, lanes, nullptr);
+      for (int done = 0; done < n_steps; done += spl) {
+        int n = std::min(spl, n_steps - done);
+        int e = launch_fast_step(b.fast_topo, o.precision, a, dt, n, o.block_size, s);
+        if (e) return fail(GP_ERR_DEVICE, "fast kernel launch: %s", hipGetErrorString((hipError_t)e));
+        ++*launches;
+      }
+    } else {
+      if (o.precision == GP_PREC_MIXED && o.kernel == GP_KERNEL_PLAIN)
+        return fail(GP_ERR_UNSUPPORTED, "the plain kernel computes in FP64 only");
+      int st = bucket_ensure_scratch(b);
+      if (st) return st;
+      BucketArgs a = bucket_args(b, 1, nullptr);
+      for (int k = 0; k < n_steps; ++k) {
+        int e = launch_plain_step(a, dt, integrator, 0, s);
+        if (e) return fail(GP_ERR_DEVICE, "plain kernel launch: %s", hipGetErrorString((hipError_t)e));
+        ++*launches;
+      }
+    }
+  }
+  return GP_OK;
+}
+
+extern "C" {
+
+const char* gp_last_error(void) { return g_error.c_str(); }
+int gp_abi_version(void) { return GP_ABI_VERSION; }
+
+int gp_model_create(const gp_model_desc* desc, gp_model** out) {
+  try {
+    return model_create(desc, out);
+  } catch (const std::bad_alloc&) {
+    return fail(GP_ERR_NOMEM, "out of host memory");
+  } catch (...) {
+    return fail(GP_ERR_INVALID, "unexpected exception");
+  }
+}
+
+void gp_model_destroy(gp_model* model) { delete model; }
+
+int gp_model_dims(const gp_model* m, int32_t* nq, int32_t* nv, int32_t* nb) {
+  if (!m) return fail(GP_ERR_INVALID, "null model");
+  if (nq) *nq = m->nq;
+  if (nv) *nv = m->nv;
+  if (nb) *nb = m->n;
+  return GP_OK;
+}
+
+int gp_model_signature(const gp_model* m, uint64_t* sig) {
+  if (!m || !sig) return fail(GP_ERR_INVALID, "null argument");
+  *sig = m->sig;
+  return GP_OK;
+}
+
+static int validate_opts(const gp_batch_opts& o) {
+  if (o.precision != GP_PREC_F64 && o.precision != GP_PREC_MIXED) return fail(GP_ERR_INVALID, "unknown precision %d", o.precision);
+  if (o.kernel < GP_KERNEL_AUTO || o.kernel > GP_KERNEL_FAST) return fail(GP_ERR_INVALID, "unknown kernel %d", o.kernel);
+  if (o.steps_per_launch < 0) return fail(GP_ERR_INVALID, "steps_per_launch < 0");
+  if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 1024 || o.block_size % 64))
+    return fail(GP_ERR_INVALID, "block_size must be a multiple of 64 in [64, 256]");
+  if (o.block_size > 256) return fail(GP_ERR_INVALID, "block_size above the kernels' launch bound of 256");
+  if (o.lanes_per_mech != 0 && o.lanes_per_mech != 1 && o.lanes_per_mech != 2 && o.lanes_per_mech != 4 &&
+      o.lanes_per_mech != 8)
+    return fail(GP_ERR_INVALID, "lanes_per_mech must be 1, 2, 4 or 8");
+  return GP_OK;
+}
+
+int gp_batch_create(gp_model* const* models, const int64_t* counts, int32_t n_models, const gp_batch_opts* opts,
+                    gp_batch** out) {
+  if (!models || !counts || !out || n_models < 1) return fail(GP_ERR_INVALID, "null argument or no models");
+  try {
+    std::unique_ptr<gp_batch> bt(new gp_batch());
+    if (opts) bt->opts = *opts;
+    int st = validate_opts(bt->opts);
+    if (st) return st;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(GP_ERR_DEVICE, "no HIP device");
+    // group instances by topology signature (first-seen order)
+    std::map<uint64_t, int> sig_bucket;
+    std::vector<std::vector<const gp_model*>> lanes;
+    std::vector<std::vector<int64_t>> inst_of_lane;
+    int64_t inst = 0;
+    for (int k = 0; k < n_models; ++k) {
+      if (!models[k]) return fail(GP_ERR_INVALID, "model %d is null", k);
+      if (counts[k] < 0) return fail(GP_ERR_INVALID, "count %d is negative", k);
+      auto it = sig_bucket.find(models[k]->sig);
+      int bi;
+      if (it == sig_bucket.end()) {
+        bi = (int)lanes.size();
+        sig_bucket[models[k]->sig] = bi;
+        lanes.emplace_back();
+        inst_of_lane.emplace_back();
+      } else {
+        bi = it->second;
+      }
+      for (int64_t c = 0; c < counts[k]; ++c) {
+        lanes[bi].push_back(models[k]);
+        inst_of_lane[bi].push_back(inst++);
+      }
+    }
+    if (inst == 0) return fail(GP_ERR_INVALID, "the batch has no instances");
+    if (inst > (int64_t)1 << 31) return fail(GP_ERR_INVALID, "too many instances");
+    bt->n_inst = inst;
+    bt->inst_bucket.assign(inst, -1);
+    bt->inst_lane.assign(inst, -1);
+    bt->buckets.resize(lanes.size());
+    for (size_t bi = 0; bi < lanes.size(); ++bi) {
+      if (lanes[bi].empty()) continue;
+      Bucket& b = bt->buckets[bi];
+      b.inst = inst_of_lane[bi];
+      if ((st = bucket_build(b, lanes[bi]))) return st;
+      for (int l = 0; l < b.B; ++l) { bt->inst_bucket[b.inst[l]] = (int)bi; bt->inst_lane[b.inst[l]] = l; }
+    }
+    // drop empty buckets (all counts 0 for a signature)
+    bt->buckets.erase(std::remove_if(bt->buckets.begin(), bt->buckets.end(), [](const Bucket& b) { return b.B == 0; }),
+                      bt->buckets.end());
+    for (size_t bi = 0; bi < bt->buckets.size(); ++bi)
+      for (int l = 0; l < bt->buckets[bi].B; ++l) bt->inst_bucket[bt->buckets[bi].inst[l]] = (int)bi;
+    bt->qoff.assign(inst + 1, 0);
+    bt->voff.assign(inst + 1, 0);
+    for (int64_t i = 0; i < inst; ++i) {
+      const Bucket& b = bt->buckets[bt->inst_bucket[i]];
+      bt->qoff[i + 1] = bt->qoff[i] + b.nq;
+      bt->voff[i + 1] = bt->voff[i] + b.nv;
+    }
+    HIP_TRY(hipStreamCreate(&bt->stream));
+    HIP_TRY(hipEventCreate(&bt->ev0));
+    HIP_TRY(hipEventCreate(&bt->ev1));
+    *out = bt.release();
+    return GP_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(GP_ERR_NOMEM, "out of host memory");
+  }
+}
+
+void gp_batch_destroy(gp_batch* bt) {
+  if (!bt) return;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (bt->stream) (void)hipStreamDestroy(bt->stream);
+  if (bt->ev0) (void)hipEventDestroy(bt->ev0);
+  if (bt->ev1) (void)hipEventDestroy(bt->ev1);
+  delete bt;
+}
+
+int gp_batch_info(const gp_batch* bt, int64_t* n, int64_t* nq, int64_t* nv, int32_t* nb) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  if (n) *n = bt->n_inst;
+  if (nq) *nq = bt->qoff.back();
+  if (nv) *nv = bt->voff.back();
+  if (nb) *nb = (int32_t)bt->buckets.size();
+  return GP_OK;
+}
+
+int gp_batch_offsets(const gp_batch* bt, int64_t* qo, int64_t* vo) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  if (qo) std::copy(bt->qoff.begin(), bt->qoff.end(), qo);
+  if (vo) std::copy(bt->voff.begin(), bt->voff.end(), vo);
+  return GP_OK;
+}
+
+int gp_batch_set_opts(gp_batch* bt, const gp_batch_opts* o) {
+  if (!bt || !o) return fail(GP_ERR_INVALID, "null argument");
+  int st = validate_opts(*o);
+  if (st) return st;
+  bt->opts = *o;
+  for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
+  bt->graphs.clear();
+  return GP_OK;
+}
+
+int gp_batch_get_opts(const gp_batch* bt, gp_batch_opts* o) {
+  if (!bt || !o) return fail(GP_ERR_INVALID, "null argument");
+  *o = bt->opts;
+  return GP_OK;
+}
+
+// host flattened (instance-major) <-> bucket SoA
+static int transfer(gp_batch* bt, const double* hq, const double* hv, double* oq, double* ov, bool to_dev,
+                     int which /*0 state, 1 tau, 2 vdot, 3 tau_tmp*/) {
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  for (Bucket& b : bt->buckets) {
+    const size_t Bp = b.Bp;
+    const int nq = which == 0 ? b.nq : 0;
+    const int nv = b.nv;
+    std::vector<double> sq((size_t)nq * Bp), sv((size_t)nv * Bp);
+    double* dq = b.d_q;
+    double* dv = which == 0 ? b.d_v : which == 1 ? b.d_tau : which == 2 ? b.d_vdot : b.d_tau_tmp;
+    if (!to_dev) {
+      if (nq) HIP_TRY(hipMemcpy(sq.data(), dq, sq.size() * sizeof(double), hipMemcpyDeviceToHost));
+      if (nv) HIP_TRY(hipMemcpy(sv.data(), dv, sv.size() * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (which == 0 && b.B < (int)Bp) {
+      // padding lanes keep a valid zero state
+      for (int i = 0; i < b.nb; ++i)
+        if (b.dofcls[i] == 6)
+          for (size_t l = b.B; l < Bp; ++l) sq[(size_t)(b.qoff[i] + 3) * Bp + l] = 1.0;
+    }
+    for (int l = 0; l < b.B; ++l) {
+      const int64_t i = b.inst[l];
+      if (to_dev) {
+        for (int k = 0; k < nq; ++k) sq[(size_t)k * Bp + l] = hq[bt->qoff[i] + k];
+        for (int k = 0; k < nv; ++k) sv[(size_t)k * Bp + l] = hv ? hv[bt->voff[i] + k] : 0.0;
+      } else {
+        for (int k = 0; k < nq; ++k) oq[bt->qoff[i] + k] = sq[(size_t)k * Bp + l];
+        for (int k = 0; k < nv; ++k) ov[bt->voff[i] + k] = sv[(size_t)k * Bp + l];
+      }
+    }
+    if (to_dev) {
+      if (nq) HIP_TRY(hipMemcpy(dq, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice));
+      if (nv) HIP_TRY(hipMemcpy(dv, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  return GP_OK;
+}
+
+int gp_batch_set_state(gp_batch* bt, const double* q, const double* v) {
+  if (!bt || !q || !v) return fail(GP_ERR_INVALID, "null argument");
+  return transfer(bt, q, v, nullptr, nullptr, true, 0);
+}
+
+int gp_batch_get_state(gp_batch* bt, double* q, double* v) {
+  if (!bt || !q || !v) return fail(GP_ERR_INVALID, "null argument");
+  return transfer(bt, nullptr, nullptr, q, v, false, 0);
+}
+
+int gp_batch_set_tau(gp_batch* bt, const double* tau) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  return transfer(bt, nullptr, tau, nullptr, nullptr, true, 1);
+}
+
+int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* stream) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  if (!std::isfinite(dt)) return fail(GP_ERR_INVALID, "dt is not finite");
+  if (integrator < GP_SEMI_IMPLICIT_EULER || integrator > GP_RUNGE_KUTTA_4)
+    return fail(GP_ERR_INVALID, "unknown integrator %d", integrator);
+  if (n_steps < 0) return fail(GP_ERR_INVALID, "n_steps < 0");
+  hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
+  int64_t launches = 0;
+  HIP_TRY(hipEventRecord(bt->ev0, s));
+  bool graph = bt->opts.use_graph != 0 && n_steps > 0;
+  if (graph) {
+    auto key = std::make_tuple(dt, integrator, n_steps);
+    auto it = bt->graphs.find(key);
+    if (it == bt->graphs.end()) {
+      // bucket scratch must exist before capture (no allocation inside a graph)
+      for (Bucket& b : bt->buckets) {
+        int st = bucket_ensure_scratch(b);
+        if (st) return st;
+      }
+      hipGraph_t g;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      int st = batch_launch_all(bt, dt, integrator, n_steps, s, &launches);
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (st) return st;
+      if (e != hipSuccess) return fail(GP_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
+      hipGraphExec_t ex;
+      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(GP_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e));
+      it = bt->graphs.emplace(key, ex).first;
+      bt->graph_launches[key] = launches;
+    }
+    launches = bt->graph_launches[key];
+    HIP_TRY(hipGraphLaunch(it->second, s));
+  } else {
+    int st = batch_launch_all(bt, dt, integrator, n_steps, s, &launches);
+    if (st) return st;
+  }
+  HIP_TRY(hipEventRecord(bt->ev1, s));
+  bt->last_launches = launches;
+  bt->timing_valid = true;
+  return GP_OK;
+}
+
+int gp_batch_sync(gp_batch* bt) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  HIP_TRY(hipDeviceSynchronize());
+  return GP_OK;
+}
+
+int gp_batch_get_vdot(gp_batch* bt, double* vdot) {
+  if (!bt || !vdot) return fail(GP_ERR_INVALID, "null argument");
+  return transfer(bt, nullptr, nullptr, nullptr, vdot, false, 2);
+}
+
+int gp_dynamics(gp_batch* bt, const double* tau, double* vdot) {
+  if (!bt || !vdot) return fail(GP_ERR_INVALID, "null argument");
+  int st;
+  if (tau && (st = transfer(bt, nullptr, tau, nullptr, nullptr, true, 3))) return st;
+  for (Bucket& b : bt->buckets) {
+    if ((st = bucket_ensure_scratch(b))) return st;
+    BucketArgs a = bucket_args(b, 1, tau ? b.d_tau_tmp : nullptr);
+    int e = launch_plain_step(a, 0.0, GP_SEMI_IMPLICIT_EULER, 1, bt->stream);
+    if (e) return fail(GP_ERR_DEVICE, "dynamics launch: %s", hipGetErrorString((hipError_t)e));
+  }
+  return transfer(bt, nullptr, nullptr, nullptr, vdot, false, 2);
+}
+
+int gp_simulate(gp_batch* bt, double final_time, double dt, gp_control_fn control_fn, void* user, int32_t integrator,
+                double* qs, double* vs, int64_t max_records, int64_t* n_records) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  if (!std::isfinite(final_time) || !std::isfinite(dt)) return fail(GP_ERR_INVALID, "non-finite time");
+  if (!(dt > 0) && final_time > 0) return fail(GP_ERR_INVALID, "dt must be positive");
+  const int64_t nq = bt->qoff.back(), nv = bt->voff.back();
+  std::vector<double> q(std::max<int64_t>(1, nq)), v(std::max<int64_t>(1, nv)), tau(std::max<int64_t>(1, nv), 0.0);
+  int st;
+  if ((st = gp_batch_get_state(bt, q.data(), v.data()))) return st;
+  int64_t rec = 0;
+  auto record = [&]() {
+    if (rec < max_records) {
+      if (qs) std::copy(q.begin(), q.begin() + nq, qs + rec * nq);
+      if (vs) std::copy(v.begin(), v.begin() + nv, vs + rec * nv);
+    }
+    ++rec;
+  };
+  record();
+  if (!control_fn && (st = gp_batch_set_tau(bt, nullptr))) return st;
+  double t = 0.0;
+  while (t < final_time) {  // simulate.rs:102-109
+    if (control_fn) {
+      if (control_fn(user, q.data(), v.data(), t, tau.data()) != 0) return fail(GP_ERR_INVALID, "control_fn failed");
+      if ((st = gp_batch_set_tau(bt, tau.data()))) return st;
+    }
+    if ((st = gp_step(bt, dt, integrator, 1, nullptr))) return st;
+    if ((st = gp_batch_get_state(bt, q.data(), v.data()))) return st;
+    record();
+    t += dt;
+  }
+  if (n_records) *n_records = rec;
+  return GP_OK;
+}
+
+int gp_batch_faults(gp_batch* bt, uint8_t* faults) {
+  if (!bt || !faults) return fail(GP_ERR_INVALID, "null argument");
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  for (Bucket& b : bt->buckets) {
+    std::vector<int> f(b.Bp);
+    HIP_TRY(hipMemcpy(f.data(), b.d_fault, sizeof(int) * b.Bp, hipMemcpyDeviceToHost));
+    for (int l = 0; l < b.B; ++l) faults[b.inst[l]] = f[l] ? 1 : 0;
+  }
+  return GP_OK;
+}
+
+int gp_batch_clear_faults(gp_batch* bt) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  for (Bucket& b : bt->buckets) HIP_TRY(hipMemset(b.d_fault, 0, sizeof(int) * b.Bp));
+  return GP_OK;
+}
+
+int gp_batch_last_timing(const gp_batch* bt, double* ms, int64_t* launches) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  if (!bt->timing_valid) return fail(GP_ERR_INVALID, "no gp_step has run on this batch");
+  HIP_TRY(hipEventSynchronize(bt->ev1));
+  float f = 0;
+  HIP_TRY(hipEventElapsedTime(&f, bt->ev0, bt->ev1));
+  if (ms) *ms = f;
+  if (launches) *launches = bt->last_launches;
+  return GP_OK;
+}
+
+}  // extern "C"

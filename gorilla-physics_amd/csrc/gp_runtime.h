@@ -1,0 +1,87 @@
+// gp_runtime.h — host-side objects behind the opaque C handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/gorilla_amd.h"
+#include "gp_internal.h"
+
+// A validated, immutable mechanism with its parameters already in aligned
+// body frames (see gp_internal.h).
+struct gp_model {
+  int n = 0, nq = 0, nv = 0, nc = 0, nh = 0;
+  double g = 9.81;
+  std::vector<int> parent, dofcls, kind, qoff, voff, cstart;
+  std::vector<double> bp;  // [n][NBF]
+  std::vector<double> cp;  // [nc][NCF], sorted by body
+  std::vector<double> hp;  // [nh][NHF]
+  std::vector<int> contact_order;  // sorted slot -> index in the description
+  uint64_t sig = 0;
+  int fast_topo = -1;
+};
+
+namespace gp {
+
+struct Bucket {
+  int nb = 0, nq = 0, nv = 0, nc = 0, nh = 0, B = 0, Bp = 0, fast_topo = -1;
+  double g = 9.81;
+  std::vector<int> parent, qoff, voff, dofcls, cstart;
+  std::vector<int64_t> inst;  // lane -> instance index
+  int *d_parent = nullptr, *d_qoff = nullptr, *d_voff = nullptr, *d_cstart = nullptr, *d_kind = nullptr;
+  double *d_bp = nullptr, *d_cp = nullptr, *d_hp = nullptr;
+  float *d_bpf = nullptr, *d_cpf = nullptr, *d_hpf = nullptr;
+  double *d_q = nullptr, *d_v = nullptr, *d_tau = nullptr, *d_tau_tmp = nullptr, *d_vdot = nullptr;
+  int* d_fault = nullptr;
+  int* d_active = nullptr;
+  double* d_scratch = nullptr;
+  Bucket() = default;
+  Bucket(const Bucket&) = delete;
+  Bucket& operator=(const Bucket&) = delete;
+  Bucket(Bucket&& o) noexcept { *this = std::move(o); }
+  Bucket& operator=(Bucket&& o) noexcept {
+    if (this != &o) {
+      release();
+      nb = o.nb; nq = o.nq; nv = o.nv; nc = o.nc; nh = o.nh; B = o.B; Bp = o.Bp; fast_topo = o.fast_topo; g = o.g;
+      parent = std::move(o.parent); qoff = std::move(o.qoff); voff = std::move(o.voff); dofcls = std::move(o.dofcls);
+      cstart = std::move(o.cstart); inst = std::move(o.inst);
+      d_parent = o.d_parent; d_qoff = o.d_qoff; d_voff = o.d_voff; d_cstart = o.d_cstart; d_kind = o.d_kind;
+      d_bp = o.d_bp; d_cp = o.d_cp; d_hp = o.d_hp; d_bpf = o.d_bpf; d_cpf = o.d_cpf; d_hpf = o.d_hpf;
+      d_q = o.d_q; d_v = o.d_v; d_tau = o.d_tau; d_tau_tmp = o.d_tau_tmp; d_vdot = o.d_vdot; d_fault = o.d_fault;
+      d_active = o.d_active; d_scratch = o.d_scratch;
+      o.d_parent = o.d_qoff = o.d_voff = o.d_cstart = o.d_kind = o.d_fault = o.d_active = nullptr;
+      o.d_bp = o.d_cp = o.d_hp = o.d_q = o.d_v = o.d_tau = o.d_tau_tmp = o.d_vdot = o.d_scratch = nullptr;
+      o.d_bpf = o.d_cpf = o.d_hpf = nullptr;
+    }
+    return *this;
+  }
+  ~Bucket();
+  void release();
+};
+
+int fail(int code, const char* fmt, ...);
+int model_create(const gp_model_desc* d, gp_model** out);
+int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models);
+int bucket_ensure_scratch(Bucket& b);
+BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau);
+
+}  // namespace gp
+
+struct gp_batch {
+  std::vector<gp::Bucket> buckets;
+  int64_t n_inst = 0;
+  std::vector<int64_t> qoff, voff;
+  std::vector<int> inst_bucket, inst_lane;
+  gp_batch_opts opts{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timing_valid = false;
+  int64_t last_launches = 0;
+  std::map<std::tuple<double, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<double, int, int>, int64_t> graph_launches;
+};

@@ -1,0 +1,219 @@
+/*
+ * gorilla_amd.h — C ABI of the MI355X batched rigid-body step.
+ *
+ * The reference crate (one-for-all/gorilla-physics, Rust) exposes this path
+ * in-process through MechanismState / step / simulate / dynamics_continuous.
+ * Each entry point below replaces one of those and names the reference
+ * interface it stands in for (file:line inside the reference crate).
+ *
+ * Conventions (identical to the reference):
+ *  - body i is the child of joint i; parent[i] < i; parent -1 is the world
+ *    (mechanism.rs:37-46, 98-116);
+ *  - x_tree is the joint's initial transform successor -> predecessor
+ *    (init_iso: revolute.rs:18, prismatic.rs:23, floating.rs:9);
+ *  - spatial vectors are [angular; linear]; floating-joint velocity is the body
+ *    twist relative to the parent, expressed in the body frame
+ *    (joint/mod.rs:295-296);
+ *  - flattening of q per instance: revolute/prismatic 1 value, floating 7
+ *    values [qx,qy,qz,qw, tx,ty,tz], fixed 0 (joint/mod.rs:208-224);
+ *    v / tau / vdot: 1, 6 ([w; v]) and 0 values (joint/mod.rs:286-303);
+ *  - gravity is a base acceleration +g z (dynamics.rs:212-218, lib.rs:39).
+ *
+ * Threading: a handle is not thread-safe; distinct handles may be used from
+ * distinct threads. Every call returns an int status (GP_OK = 0); no C++
+ * exception or abort crosses this boundary. gp_last_error() returns a
+ * thread-local message for the last failing call on the calling thread.
+ */
+#ifndef GORILLA_AMD_H
+#define GORILLA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_ABI_VERSION 1
+
+/* ---- status codes (replace the reference's panic!/expect sites) ---------- */
+enum {
+  GP_OK = 0,
+  GP_ERR_INVALID = 1,     /* bad argument (NULL, size, non-unit axis ...)     */
+  GP_ERR_TOPOLOGY = 2,    /* parent >= own index, unknown joint type          */
+  GP_ERR_NOMEM = 3,       /* host or device allocation failed                 */
+  GP_ERR_DEVICE = 4,      /* HIP runtime error / no device                    */
+  GP_ERR_UNSUPPORTED = 5, /* option not available for this batch              */
+  GP_ERR_FAULT = 6,       /* an instance produced a non-finite value / D <= 0 */
+  GP_ERR_BUSY = 7         /* server full (no slot / memory for the request)   */
+};
+
+/* ---- joint types (joint/mod.rs:22) ---------------------------------------- */
+enum {
+  GP_JOINT_FIXED = 0,
+  GP_JOINT_REVOLUTE = 1,
+  GP_JOINT_PRISMATIC = 2,
+  GP_JOINT_FLOATING = 3
+};
+
+/* ---- integrators (integrators.rs:17) -------------------------------------- */
+enum {
+  GP_SEMI_IMPLICIT_EULER = 0,
+  GP_RUNGE_KUTTA_2 = 1,
+  GP_RUNGE_KUTTA_4 = 2
+};
+
+/* ---- arithmetic of the batched step --------------------------------------- */
+enum {
+  /* everything in FP64 (the reference's Float = f64, types.rs:3) */
+  GP_PREC_F64 = 0,
+  /* FP32 kinematics, bias and contact forces and both ABA sweeps; FP64 for the
+   * D = S^T I^A S solve and for the q/v accumulation of the integrator */
+  GP_PREC_MIXED = 1
+};
+
+/* ---- kernel family --------------------------------------------------------- */
+enum {
+  GP_KERNEL_AUTO = 0,  /* FAST when the bucket topology has a compiled kernel */
+  GP_KERNEL_PLAIN = 1, /* runtime-topology FP64 kernel, one step per launch,
+                          per-body intermediates in device scratch            */
+  GP_KERNEL_FAST = 2   /* compile-time topology, register resident, several
+                          steps per launch                                     */
+};
+
+/* One body and the joint that connects it to its parent. */
+typedef struct gp_body_desc {
+  int32_t parent;          /* -1 = world, else index of the parent body (< own) */
+  int32_t jtype;           /* GP_JOINT_*                                        */
+  double axis[3];          /* joint axis, successor frame (revolute/prismatic)  */
+  double x_tree_quat[4];   /* init_iso rotation as quaternion [x,y,z,w]         */
+  double x_tree_trans[3];  /* init_iso translation                              */
+  double moment[6];        /* J about the body-frame origin: xx,yy,zz,xy,xz,yz   */
+  double mc[3];            /* mass * centre of mass (cross_part, inertia.rs:32) */
+  double mass;
+  double spring_k;         /* prismatic joint spring (k = 0: none), prismatic.rs:13 */
+  double spring_l;         /* spring rest length                                */
+} gp_body_desc;
+
+/* Contact point on a body (contact.rs:17); default k is 50e3 (contact.rs:28). */
+typedef struct gp_contact_desc {
+  int32_t body;
+  int32_t _pad;
+  double p[3];  /* location in the body frame */
+  double k;
+} gp_contact_desc;
+
+/* Half-space (collision/halfspace.rs:6); defaults alpha 0.9, mu 0.5. */
+typedef struct gp_halfspace_desc {
+  double point[3];
+  double normal[3]; /* unit outward normal */
+  double alpha;
+  double mu;
+} gp_halfspace_desc;
+
+typedef struct gp_model_desc {
+  int32_t n_bodies;
+  int32_t n_contacts;
+  int32_t n_halfspaces;
+  int32_t _pad;
+  const gp_body_desc* bodies;
+  const gp_contact_desc* contacts;
+  const gp_halfspace_desc* halfspaces;
+  double gravity; /* 9.81 in the reference (lib.rs:39) */
+} gp_model_desc;
+
+/* Options of a batch. Zero-initialised = defaults. */
+typedef struct gp_batch_opts {
+  int32_t precision;         /* GP_PREC_*                                     */
+  int32_t kernel;            /* GP_KERNEL_*                                   */
+  int32_t steps_per_launch;  /* FAST kernels: steps fused in one launch (0 = 1) */
+  int32_t block_size;        /* threads per workgroup (0 = 256)               */
+  int32_t lanes_per_mech;    /* lanes cooperating on one instance (0 = 1)     */
+  int32_t use_graph;         /* capture multi-launch rollouts in a hipGraph   */
+  int32_t _reserved[2];
+} gp_batch_opts;
+
+typedef struct gp_model gp_model;
+typedef struct gp_batch gp_batch;
+
+/* ---- errors ------------------------------------------------------------------ */
+const char* gp_last_error(void);
+int gp_abi_version(void);
+
+/* ---- model --------------------------------------------------------------------
+ * Replaces MechanismState::new (mechanism.rs:62) with add_halfspace (:379) and
+ * add_contact_point (:384). Validation that panics in the reference
+ * (mechanism.rs:91-115) returns GP_ERR_TOPOLOGY here. The model is immutable. */
+int gp_model_create(const gp_model_desc* desc, gp_model** out);
+void gp_model_destroy(gp_model* model);
+/* nq, nv of one instance (flattening of joint/mod.rs:208-303) */
+int gp_model_dims(const gp_model* model, int32_t* nq, int32_t* nv, int32_t* n_bodies);
+/* Topology signature used for bucketing (parents, dof classes, contacts). */
+int gp_model_signature(const gp_model* model, uint64_t* sig);
+
+/* ---- batch ----------------------------------------------------------------------
+ * counts[k] instances of models[k]; instances are numbered in that order. Every
+ * instance starts at the reference's zero state (q = 0, floating q = identity,
+ * v = 0; mechanism.rs:68-88) with zero held torque. State lives in device memory
+ * as structure-of-arrays per topology bucket. opts may be NULL. */
+int gp_batch_create(gp_model* const* models, const int64_t* counts, int32_t n_models,
+                    const gp_batch_opts* opts, gp_batch** out);
+void gp_batch_destroy(gp_batch* batch);
+int gp_batch_info(const gp_batch* batch, int64_t* n_instances, int64_t* nq_total,
+                  int64_t* nv_total, int32_t* n_buckets);
+/* per-instance offsets into the flattened host arrays (n_instances + 1 each) */
+int gp_batch_offsets(const gp_batch* batch, int64_t* q_offsets, int64_t* v_offsets);
+int gp_batch_set_opts(gp_batch* batch, const gp_batch_opts* opts);
+int gp_batch_get_opts(const gp_batch* batch, gp_batch_opts* opts);
+
+/* MechanismState::update(q, v) (mechanism.rs:209) for every instance; host
+ * arrays, instance-major flattening. */
+int gp_batch_set_state(gp_batch* batch, const double* q, const double* v);
+int gp_batch_get_state(gp_batch* batch, double* q, double* v);
+/* Torques held over the following steps (the tau argument of step(),
+ * simulate.rs:20); NULL = zeros, as an empty tau in simulate.rs:27-37. */
+int gp_batch_set_tau(gp_batch* batch, const double* tau);
+
+/* step(state, dt, tau, integrator) (simulate.rs:20) applied n_steps times with
+ * the held torques. Asynchronous on `stream` (a hipStream_t; NULL = the batch's
+ * own stream). */
+int gp_step(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, void* stream);
+/* Block until the batch's work on its stream is done. */
+int gp_batch_sync(gp_batch* batch);
+
+/* Joint accelerations of the last step's first dynamics evaluation (the vdot
+ * the reference's semi_implicit_euler computes, integrators.rs:30), host array
+ * of nv_total values. */
+int gp_batch_get_vdot(gp_batch* batch, double* vdot);
+
+/* dynamics_continuous(state, tau) (dynamics.rs:322): vdot for every instance at
+ * the current state. tau NULL = held torques. Host arrays, synchronous. */
+int gp_dynamics(gp_batch* batch, const double* tau, double* vdot);
+
+/* simulate(state, final_time, dt, control_fn, integrator) (simulate.rs:87).
+ * The loop is the reference's float-accumulated `while t < final_time`
+ * (simulate.rs:97-109). control_fn (may be NULL = zero torque) is called once
+ * per step on the host with the current state and fills tau (nv_total values).
+ * If qs / vs are non-NULL they receive the initial state and every step's state,
+ * (max_records rows of nq_total / nv_total values); n_records gets the number
+ * of rows the loop produced (steps + 1). */
+typedef int (*gp_control_fn)(void* user, const double* q, const double* v, double t,
+                             double* tau_out);
+int gp_simulate(gp_batch* batch, double final_time, double dt, gp_control_fn control_fn,
+                void* user, int32_t integrator, double* qs, double* vs, int64_t max_records,
+                int64_t* n_records);
+
+/* Per-instance fault flags (1 = a non-finite q/v/vdot or a non-positive D was
+ * seen since the last clear). No reference counterpart: the reference panics
+ * (dynamics.rs:267-274). */
+int gp_batch_faults(gp_batch* batch, uint8_t* faults);
+int gp_batch_clear_faults(gp_batch* batch);
+
+/* Time the batch's last gp_step on its own stream (HIP events around every
+ * launch): total milliseconds and number of kernel launches. */
+int gp_batch_last_timing(const gp_batch* batch, double* ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GORILLA_AMD_H */

@@ -1,0 +1,222 @@
+// oracle.cpp — TEST INFRASTRUCTURE ONLY: C entry points of the CPU oracle
+// (oracle.hpp) for the pytest parity suite, smoke() and bench.py's cpu_baseline.
+// The product library never links or loads this file.
+#include <algorithm>
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+
+enum { M_CRBA = 0, M_ABA = 1, M_CRBA_LD = 2, M_ABA_LD = 3 };
+
+template <class T>
+int dyn_any(const Model<T>& M, const AbaModel<T>* A, int method, const T* q, const T* v, const T* tau, T* vdot) {
+  if (method == M_ABA || method == M_ABA_LD) return aba_dynamics(*A, q, v, tau, vdot);
+  return crba_dynamics(M, q, v, tau, vdot);
+}
+
+template <class T>
+int step_t(const gp_model_desc* d, int integrator, int method, const double* q, const double* v,
+           const double* tau, double dt, double* q_out, double* v_out, double* vdot_out) {
+  Model<T> M = load_model<T>(*d);
+  AbaModel<T> A = make_aba_model(M);
+  std::vector<T> qt(M.nq), vt(M.nv), tt(M.nv), qo(M.nq), vo(M.nv), ao(M.nv);
+  for (int k = 0; k < M.nq; ++k) qt[k] = T(q[k]);
+  for (int k = 0; k < M.nv; ++k) { vt[k] = T(v[k]); tt[k] = tau ? T(tau[k]) : T(0); }
+  auto dyn = [&](const Model<T>& MM, const T* qq, const T* vv, const T* ta, T* out) {
+    return dyn_any(MM, &A, method, qq, vv, ta, out);
+  };
+  int st = step_integrate(M, integrator, qt.data(), vt.data(), tt.data(), T(dt), qo.data(), vo.data(), ao.data(), dyn);
+  for (int k = 0; k < M.nq; ++k) if (q_out) q_out[k] = double(qo[k]);
+  for (int k = 0; k < M.nv; ++k) { if (v_out) v_out[k] = double(vo[k]); if (vdot_out) vdot_out[k] = double(ao[k]); }
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_dims(const gp_model_desc* d, int* nq, int* nv) {
+  Model<double> M = load_model<double>(*d);
+  *nq = M.nq; *nv = M.nv;
+  return 0;
+}
+
+// dynamics_continuous (dynamics.rs:322). method: 0 CRBA f64, 1 ABA f64,
+// 2 CRBA long double, 3 ABA long double.
+int oracle_dynamics(const gp_model_desc* d, const double* q, const double* v, const double* tau,
+                    double* vdot, int method) {
+  if (method >= M_CRBA_LD) {
+    Model<long double> M = load_model<long double>(*d);
+    AbaModel<long double> A = make_aba_model(M);
+    std::vector<long double> qt(M.nq), vt(M.nv), tt(M.nv), ao(M.nv);
+    for (int k = 0; k < M.nq; ++k) qt[k] = q[k];
+    for (int k = 0; k < M.nv; ++k) { vt[k] = v[k]; tt[k] = tau ? tau[k] : 0.0; }
+    int st = dyn_any(M, &A, method, qt.data(), vt.data(), tt.data(), ao.data());
+    for (int k = 0; k < M.nv; ++k) vdot[k] = double(ao[k]);
+    return st;
+  }
+  Model<double> M = load_model<double>(*d);
+  AbaModel<double> A = make_aba_model(M);
+  std::vector<double> tt(M.nv, 0.0);
+  if (tau) for (int k = 0; k < M.nv; ++k) tt[k] = tau[k];
+  return dyn_any(M, &A, method, q, v, tt.data(), vdot);
+}
+
+// mass matrix (mechanism.rs:637) and bias c(q,v) - tau_contact (dynamics.rs:233)
+int oracle_mass_matrix(const gp_model_desc* d, const double* q, const double* v, double* Mout, double* cout) {
+  Model<double> M = load_model<double>(*d);
+  std::vector<double> tau(M.nv, 0.0), vd(M.nv);
+  return crba_dynamics(M, q, v, tau.data(), vd.data(), Mout, cout);
+}
+
+// one step() (simulate.rs:20) of the given integrator
+int oracle_step(const gp_model_desc* d, int integrator, int method, const double* q, const double* v,
+                const double* tau, double dt, double* q_out, double* v_out, double* vdot_out) {
+  if (method >= M_CRBA_LD) return step_t<long double>(d, integrator, method, q, v, tau, dt, q_out, v_out, vdot_out);
+  return step_t<double>(d, integrator, method, q, v, tau, dt, q_out, v_out, vdot_out);
+}
+
+// In-place rollout of n_inst instances (instance i uses descs[i]) over n_steps
+// steps with held torques, on n_threads host threads (one instance per thread
+// at a time). State arrays are flattened instance-major. Returns the number of
+// instances whose solve failed.
+int oracle_rollout(const gp_model_desc* const* descs, int64_t n_inst, double* q, double* v,
+                   const double* tau, double dt, int integrator, int method, int64_t n_steps,
+                   int n_threads) {
+  struct Prep { Model<double> M; AbaModel<double> A; };
+  std::map<const gp_model_desc*, int> idx;
+  std::vector<Prep> preps;
+  std::vector<int> which(n_inst);
+  std::vector<int64_t> qo(n_inst + 1, 0), vo(n_inst + 1, 0);
+  for (int64_t i = 0; i < n_inst; ++i) {
+    auto it = idx.find(descs[i]);
+    int k;
+    if (it == idx.end()) {
+      Prep p; p.M = load_model<double>(*descs[i]); p.A = make_aba_model(p.M);
+      k = int(preps.size()); preps.push_back(std::move(p)); idx[descs[i]] = k;
+    } else k = it->second;
+    which[i] = k;
+    qo[i + 1] = qo[i] + preps[k].M.nq;
+    vo[i + 1] = vo[i] + preps[k].M.nv;
+  }
+  std::atomic<int64_t> next(0);
+  std::atomic<int> failures(0);
+  auto worker = [&]() {
+    std::vector<double> qn, vn, zero;
+    for (;;) {
+      int64_t i = next.fetch_add(1);
+      if (i >= n_inst) break;
+      const Prep& P = preps[which[i]];
+      double* qi = q + qo[i];
+      double* vi = v + vo[i];
+      zero.assign(P.M.nv, 0.0);
+      const double* ti = tau ? tau + vo[i] : zero.data();
+      qn.resize(P.M.nq); vn.resize(P.M.nv);
+      auto dyn = [&](const Model<double>& MM, const double* qq, const double* vv, const double* ta, double* out) {
+        return dyn_any(MM, &P.A, method, qq, vv, ta, out);
+      };
+      for (int64_t s = 0; s < n_steps; ++s) {
+        if (step_integrate(P.M, integrator, qi, vi, ti, dt, qn.data(), vn.data(), (double*)nullptr, dyn)) {
+          failures.fetch_add(1);
+          break;
+        }
+        std::copy(qn.begin(), qn.end(), qi);
+        std::copy(vn.begin(), vn.end(), vi);
+      }
+    }
+  };
+  n_threads = std::max(1, n_threads);
+  std::vector<std::thread> pool;
+  for (int t = 1; t < n_threads; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& t : pool) t.join();
+  return failures.load();
+}
+
+// simulate() (simulate.rs:87-112): float-accumulated loop `while t < final_time`,
+// recording every state. Constant torque (NULL = zeros).
+int oracle_simulate(const gp_model_desc* d, double final_time, double dt, int integrator, int method,
+                    const double* q0, const double* v0, const double* tau, double* qs, double* vs,
+                    int64_t max_records, int64_t* n_records) {
+  Model<double> M = load_model<double>(*d);
+  AbaModel<double> A = make_aba_model(M);
+  std::vector<double> q(q0, q0 + M.nq), v(v0, v0 + M.nv), qn(M.nq), vn(M.nv), t0(M.nv, 0.0);
+  const double* ta = tau ? tau : t0.data();
+  auto dyn = [&](const Model<double>& MM, const double* qq, const double* vv, const double* tt, double* out) {
+    return dyn_any(MM, &A, method, qq, vv, tt, out);
+  };
+  int64_t rec = 0;
+  auto record = [&]() {
+    if (rec < max_records) {
+      if (qs) std::copy(q.begin(), q.end(), qs + rec * M.nq);
+      if (vs) std::copy(v.begin(), v.end(), vs + rec * M.nv);
+    }
+    ++rec;
+  };
+  record();
+  double t = 0.0;
+  while (t < final_time) {
+    int st = step_integrate(M, integrator, q.data(), v.data(), ta, dt, qn.data(), vn.data(), (double*)nullptr, dyn);
+    if (st) { *n_records = rec; return st; }
+    q = qn; v = vn;
+    record();
+    t += dt;
+  }
+  *n_records = rec;
+  return 0;
+}
+
+int oracle_energy(const gp_model_desc* d, const double* q, const double* v, double* ke, double* pe) {
+  Model<double> M = load_model<double>(*d);
+  energy(M, q, v, ke, pe);
+  return 0;
+}
+
+// signed distance (p - h.point) . n of every (contact point, half-space) pair,
+// contact-major in the order of desc->contacts (halfspace.rs:46)
+int oracle_contact_distances(const gp_model_desc* d, const double* q, double* dist) {
+  Model<double> M = load_model<double>(*d);
+  std::vector<Iso<double>> b2r(M.n);
+  for (int i = 0; i < M.n; ++i) {
+    Iso<double> XJ = joint_transform(M, i, q);
+    b2r[i] = M.parent[i] < 0 ? XJ : imul(b2r[M.parent[i]], XJ);
+  }
+  for (int c = 0; c < d->n_contacts; ++c) {
+    const gp_contact_desc& cd = d->contacts[c];
+    M3<double> R = qmat(b2r[cd.body].r);
+    V3<double> pw = R * V3<double>(cd.p[0], cd.p[1], cd.p[2]) + b2r[cd.body].t;
+    for (int h = 0; h < d->n_halfspaces; ++h) {
+      const HalfSpace<double>& hs = M.halfspaces[h];
+      dist[c * d->n_halfspaces + h] = dot(pw - hs.point, hs.normal);
+    }
+  }
+  return 0;
+}
+
+// Operation counts of one step of the ABA formulation with the given
+// integrator at state (q, v): [add, mul, div, sqrt, trig, pow].
+int oracle_opcount(const gp_model_desc* d, const double* q, const double* v, const double* tau,
+                   int integrator, uint64_t* counts) {
+  Model<Cnt> M = load_model<Cnt>(*d);
+  AbaModel<Cnt> A = make_aba_model(M);
+  std::vector<Cnt> qt(M.nq), vt(M.nv), tt(M.nv), qo(M.nq), vo(M.nv), ao(M.nv);
+  for (int k = 0; k < M.nq; ++k) qt[k] = Cnt(q[k]);
+  for (int k = 0; k < M.nv; ++k) { vt[k] = Cnt(v[k]); tt[k] = Cnt(tau ? tau[k] : 0.0); }
+  auto dyn = [&](const Model<Cnt>&, const Cnt* qq, const Cnt* vv, const Cnt* ta, Cnt* out) {
+    return aba_dynamics(A, qq, vv, ta, out);
+  };
+  OpCounter::reset();
+  int st = step_integrate(M, integrator, qt.data(), vt.data(), tt.data(), Cnt(0.001), qo.data(), vo.data(), ao.data(), dyn);
+  counts[0] = OpCounter::add; counts[1] = OpCounter::mul; counts[2] = OpCounter::div;
+  counts[3] = OpCounter::sqrt_; counts[4] = OpCounter::trig; counts[5] = OpCounter::pow_;
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+"""The C-ABI library loads, exports every entry point the header declares, and
+validates models the way the reference panics (CPU only: no kernel runs)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "gorilla_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(gp_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
+
+
+def test_header_declares_the_api():
+    names = declared_symbols()
+    for must in ("gp_model_create", "gp_batch_create", "gp_step", "gp_simulate", "gp_dynamics",
+                 "gp_batch_set_state", "gp_batch_get_state", "gp_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(gp):
+    lib = gp.abi.load_library()
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes signature table covers them all
+    assert set(declared_symbols()) <= set(gp.abi.SIGNATURES)
+
+
+def _desc(gp, **over):
+    from gorilla_physics_amd import workloads
+    d = workloads.acrobot_desc()
+    for k, v in over.items():
+        setattr(d._bodies[1], k, v)
+    return d
+
+
+def _create(gp, d):
+    lib = gp.abi.load_library()
+    h = C.c_void_p()
+    code = lib.gp_model_create(d.ptr(), C.byref(h))
+    if code == 0:
+        lib.gp_model_destroy(h)
+    return code, lib.gp_last_error().decode()
+
+
+def test_model_validation_codes(gp):
+    abi = gp.abi
+    assert _create(gp, _desc(gp))[0] == abi.GP_OK
+    code, msg = _create(gp, _desc(gp, parent=1))
+    assert code == abi.GP_ERR_TOPOLOGY and "parent" in msg
+    code, msg = _create(gp, _desc(gp, jtype=7))
+    assert code == abi.GP_ERR_TOPOLOGY
+    d = _desc(gp)
+    d._bodies[1].axis[:] = [0.0, 2.0, 0.0]
+    assert _create(gp, d)[0] == abi.GP_ERR_INVALID
+    d = _desc(gp)
+    d._bodies[1].x_tree_quat[:] = [0.0, 0.0, 0.0, 2.0]
+    assert _create(gp, d)[0] == abi.GP_ERR_INVALID
+    d = _desc(gp)
+    d._bodies[1].mass = float("nan")
+    assert _create(gp, d)[0] == abi.GP_ERR_INVALID
+    d = _desc(gp)
+    d._bodies[1].mass = -1.0
+    assert _create(gp, d)[0] == abi.GP_ERR_INVALID
+
+
+def test_contact_and_halfspace_validation(gp):
+    from gorilla_physics_amd import workloads
+    d = workloads.cube_desc()
+    assert _create(gp, d)[0] == gp.abi.GP_OK
+    d = workloads.cube_desc()
+    d._contacts[3].body = 5
+    assert _create(gp, d)[0] == gp.abi.GP_ERR_INVALID
+    d = workloads.cube_desc()
+    d._halfspaces[0].normal[:] = [0.0, 0.0, 3.0]
+    assert _create(gp, d)[0] == gp.abi.GP_ERR_INVALID
+
+
+def test_null_arguments_return_codes(gp):
+    lib = gp.abi.load_library()
+    assert lib.gp_model_create(None, None) == gp.abi.GP_ERR_INVALID
+    assert lib.gp_step(None, 1e-3, 0, 1, None) == gp.abi.GP_ERR_INVALID
+    assert lib.gp_batch_set_state(None, None, None) == gp.abi.GP_ERR_INVALID
+
+
+def test_dims_and_signature(gp):
+    from gorilla_physics_amd import batch, workloads
+    w = workloads.arms(n=3)
+    ms = [batch.Model(d) for d in w["models"]]
+    assert all((m.nq, m.nv, m.n_bodies) == (7, 7, 7) for m in ms)
+    assert len({m.signature() for m in ms}) == 1  # same topology, different parameters
+    cube = batch.Model(workloads.cube_desc())
+    assert (cube.nq, cube.nv) == (7, 6)
+    assert cube.signature() != ms[0].signature()
+    wm = workloads.mixed(n=64)
+    sigs = {}
+    for d in wm["models"]:
+        m = batch.Model(d)
+        sigs.setdefault((m.n_bodies, m.nv), set()).add(m.signature())
+    assert all(len(s) == 1 for s in sigs.values())  # R / P mixes share a bucket
+
+
+def test_batch_without_device_reports_device_error(gp):
+    """On a machine without a GPU the batch cannot be created: a status code,
+    never a crash or a silent CPU fallback."""
+    import subprocess
+    import sys
+    probe = subprocess.run(["/opt/rocm/bin/rocminfo"], capture_output=True, text=True) if os.path.exists(
+        "/opt/rocm/bin/rocminfo") else None
+    if probe is not None and "gfx9" in probe.stdout:
+        pytest.skip("a GPU is present")
+    from gorilla_physics_amd import batch, workloads
+    m = batch.Model(workloads.acrobot_desc())
+    with pytest.raises(gp.abi.GPError) as e:
+        batch.Batch([m], [2])
+    assert e.value.code == gp.abi.GP_ERR_DEVICE
+
+
+def test_oracle_library_loads(oracle):
+    from gorilla_physics_amd import workloads
+    assert oracle.dims(workloads.cube_desc()) == (7, 6)

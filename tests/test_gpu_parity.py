@@ -1,0 +1,115 @@
+"""GPU parity: one batched step on the MI355X against the CPU oracle (the
+reference's CRBA + RNEA + LU restated in FP64), teacher-forced from the same
+state. Metric e(x) = max|x - x_ref| / max(max|x_ref|, 1) per instance for
+x in {q', v', qdd}; quaternions compared up to sign.
+
+Bounds (written here, stated per workload):
+  FP64 kernels (plain and fast):            e <= 1e-10
+  mixed kernels (FP32 sweeps, FP64 D + q/v): e(q'), e(v') <= 1e-6, e(qdd) <= MIXED_QDD
+"""
+import numpy as np
+import pytest
+
+from gpu_helpers import rel_err, step_errors
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-10
+MIXED_QV_TOL = 1e-6
+MIXED_QDD_TOL = 5e-3
+
+
+def _run_step(gp, w, n_pick=None, **opts):
+    from gorilla_physics_amd import batch
+    b, models = batch.batch_from_workload(w, **opts)
+    q0, v0 = b.get_state()
+    b.step(w["dt"], 0, 1)
+    q1, v1 = b.get_state()
+    a1 = b.get_vdot()
+    assert not b.faults().any()
+    return q0, v0, q1, v1, a1
+
+
+def _descs(w):
+    out = []
+    for d, c in zip(w["models"], w["counts"]):
+        out.extend([d] * int(c))
+    return out
+
+
+@pytest.fixture(scope="module")
+def wl(gp):
+    from gorilla_physics_amd import workloads
+    return dict(
+        acrobot=workloads.acrobot(n=256),
+        arms=workloads.arms(n=512),
+        mixed=workloads.mixed(n=512),
+        cubes=workloads.cubes(n=256),
+    )
+
+
+@pytest.mark.parametrize("name", ["acrobot", "arms", "mixed", "cubes"])
+@pytest.mark.parametrize("kernel", ["plain", "fast"])
+def test_f64_step_parity(gp, oracle, wl, name, kernel):
+    from gorilla_physics_amd import abi
+    w = wl[name]
+    k = abi.GP_KERNEL_PLAIN if kernel == "plain" else abi.GP_KERNEL_FAST
+    print(f"case f64 {name} {kernel}", flush=True)
+    q0, v0, q1, v1, a1 = _run_step(gp, w, precision=abi.GP_PREC_F64, kernel=k)
+    eq, ev, ea = step_errors(oracle, _descs(w), q0, v0, w["tau"], q1, v1, a1, w["dt"])
+    print(f"{name} {kernel}: max e(q') {eq.max():.3g} e(v') {ev.max():.3g} e(qdd) {ea.max():.3g}")
+    assert eq.max() <= F64_TOL and ev.max() <= F64_TOL and ea.max() <= F64_TOL
+
+
+@pytest.mark.parametrize("name", ["acrobot", "arms", "mixed", "cubes"])
+def test_mixed_step_bound(gp, oracle, wl, name):
+    from gorilla_physics_amd import abi
+    w = wl[name]
+    print(f"case mixed {name}", flush=True)
+    q0, v0, q1, v1, a1 = _run_step(gp, w, precision=abi.GP_PREC_MIXED, kernel=abi.GP_KERNEL_FAST)
+    eq, ev, ea = step_errors(oracle, _descs(w), q0, v0, w["tau"], q1, v1, a1, w["dt"])
+    print(f"{name} mixed: max e(q') {eq.max():.3g} e(v') {ev.max():.3g} e(qdd) {ea.max():.3g} "
+          f"p99 e(qdd) {np.percentile(ea, 99):.3g}")
+    assert eq.max() <= MIXED_QV_TOL and ev.max() <= MIXED_QV_TOL and ea.max() <= MIXED_QDD_TOL
+
+
+def test_reference_known_answers_on_gpu(gp):
+    """The reference's closed-form dynamics tests, run through the HIP path
+    via the MechanismState mirror (dynamics.rs:883-1088, floating.rs:82-129)."""
+    from gorilla_physics_amd import builders
+    g = gp.GRAVITY
+    m, l = 5.0, 7.0
+    rod = builders.build_pendulum(m, np.diag([0, m * l * l / 3, m * l * l / 3]), [m * l / 2, 0, 0],
+                                  gp.Isometry3.identity(), [0, 1, 0])
+    assert abs(gp.dynamics_continuous(rod, [0.0])[0] - 3 * g / (2 * l)) < 1e-12
+    rot = builders.build_pendulum(m, np.diag([0, m * l * l / 3, m * l * l / 3]), [m * l / 2, 0, 0],
+                                  gp.Isometry3.rotation_scaled_axis([np.pi / 2, 0, 0]), [0, 0, 1])
+    assert abs(gp.dynamics_continuous(rot, [0.0])[0] + 3 * g / (2 * l)) < 1e-12
+    hold = builders.build_pendulum(m, np.diag([0, m * l * l / 3, m * l * l / 3]), [m * l / 2, 0, 0],
+                                   gp.Isometry3.identity(), [0, 1, 0])
+    assert abs(gp.dynamics_continuous(hold, [-m * g * l / 2])[0]) < 1e-6
+    ball = gp.RigidBody(gp.SpatialInertia("ball", np.diag([2.0, 2.0, 2.0]), [0, 0, 0], 5.0))
+    st = gp.MechanismState([gp.FloatingJoint(gp.Transform3D.identity("ball", gp.WORLD_FRAME))], [ball])
+    st.update([gp.Pose(gp.UnitQuaternion.from_euler_angles(0.1, 0.2, 0.3), [1, 2, 3])],
+              [gp.SpatialVector([1, 2, 3], [4, 5, 6])])
+    acc = gp.dynamics_continuous(st, [gp.SpatialVector.zero()])[0]
+    got = np.concatenate([acc.angular, acc.linear])
+    assert np.allclose(got, [0, 0, 0, 4.948946, -6.959844, -6.566419], atol=1e-5)  # RigidBodyDynamics.jl
+
+
+def test_lane_independence(gp):
+    """An instance's result does not depend on its lane or its batch mates:
+    the same arm run alone and at lane 77 of a mixed batch is bit-identical."""
+    from gorilla_physics_amd import abi, batch, workloads
+    w = workloads.arms(n=128)
+    for prec in (abi.GP_PREC_F64, abi.GP_PREC_MIXED):
+        b, _ = batch.batch_from_workload(w, precision=prec, kernel=abi.GP_KERNEL_FAST)
+        b.step(w["dt"], 0, 50)
+        q_all, v_all = b.get_state()
+        i = 77
+        one = dict(models=[w["models"][i]], counts=[1], q=w["q"][7 * i:7 * i + 7], v=w["v"][7 * i:7 * i + 7],
+                   tau=w["tau"][7 * i:7 * i + 7], dt=w["dt"])
+        b1, _ = batch.batch_from_workload(one, precision=prec, kernel=abi.GP_KERNEL_FAST)
+        b1.step(w["dt"], 0, 50)
+        q1, v1 = b1.get_state()
+        assert np.array_equal(q1, q_all[7 * i:7 * i + 7]) and np.array_equal(v1, v_all[7 * i:7 * i + 7])

@@ -1,0 +1,92 @@
+"""Host-side logic of the reference-interface mirror and the workload
+generators (CPU only)."""
+import math
+
+import numpy as np
+import pytest
+
+
+def test_mechanism_state_validation_like_the_reference(gp):
+    """MechanismState::new panics when joint i's child is not body i or a
+    parent frame is unknown (mechanism.rs:91-115)."""
+    b1 = gp.RigidBody.new_sphere(1.0, 1.0, "a")
+    with pytest.raises(ValueError, match="child body"):
+        gp.MechanismState([gp.RevoluteJoint(gp.Transform3D.identity("b", gp.WORLD_FRAME), [0, 1, 0])], [b1])
+    with pytest.raises(ValueError, match="no parent body"):
+        gp.MechanismState([gp.RevoluteJoint(gp.Transform3D.identity("a", "nowhere"), [0, 1, 0])], [b1])
+
+
+def test_parents_found_by_frame_name(gp):
+    """test_supports tree of mechanism.rs:793 (bodies 1..5)."""
+    joints = [gp.FloatingJoint(gp.Transform3D.identity("1", gp.WORLD_FRAME)),
+              gp.RevoluteJoint(gp.Transform3D.move_x("2", "1", -1.0), [0, 1, 0]),
+              gp.RevoluteJoint(gp.Transform3D.move_z("3", "2", 1.0), [0, 1, 0]),
+              gp.RevoluteJoint(gp.Transform3D.move_x("4", "1", 1.0), [0, 1, 0]),
+              gp.RevoluteJoint(gp.Transform3D.move_z("5", "4", 1.0), [0, 1, 0])]
+    bodies = [gp.RigidBody.new_sphere(1.0, 1.0, str(i)) for i in range(1, 6)]
+    st = gp.MechanismState(joints, bodies)
+    assert st.parents == [0, 1, 2, 1, 4]
+    d = st.model_desc()
+    assert [d._bodies[i].parent for i in range(5)] == [-1, 0, 1, 0, 3]
+    assert (d.nq, d.nv) == (7 + 4, 6 + 4)
+
+
+def test_flattening_round_trip(gp):
+    """joint/mod.rs:208-303: Pose -> [x,y,z,w,tx,ty,tz]; spatial -> [w; v]."""
+    from gorilla_physics_amd import mechanism as m
+    joints = [gp.FloatingJoint(gp.Transform3D.identity("a", gp.WORLD_FRAME)),
+              gp.FixedJoint(gp.Transform3D.identity("b", "a")),
+              gp.RevoluteJoint(gp.Transform3D.identity("c", "b"), [0, 0, 1])]
+    q = [gp.Pose(gp.UnitQuaternion.from_axis_angle([0, 0, 1], 0.3), [1, 2, 3]), None, 0.7]
+    v = [gp.SpatialVector([1, 2, 3], [4, 5, 6]), None, -2.0]
+    fq, fv = m.flatten_q(q), m.flatten_v(v)
+    assert len(fq) == 8 and len(fv) == 7
+    assert fq[3] == pytest.approx(math.cos(0.15)) and fq[4:7] == [1, 2, 3]
+    q2, v2 = m.unflatten_q(fq, joints), m.unflatten_v(fv, joints)
+    assert q2[1] is None and v2[1] is None and q2[2] == 0.7 and list(v2[0].linear) == [4, 5, 6]
+
+
+def test_tau_length_assertion(gp):
+    """simulate.rs:39-45 asserts len(tau) == len(v); empty tau means zeros."""
+    from gorilla_physics_amd import builders, mechanism
+    st = builders.build_cube(1.0, 1.0)
+    assert np.all(mechanism._tau_flat(st, []) == 0) and len(mechanism._tau_flat(st, [])) == 6
+    with pytest.raises(AssertionError):
+        mechanism._tau_flat(st, [0.0, 0.0])
+
+
+def test_euler_angles_match_rotation_composition(gp):
+    """from_euler_angles(r, p, y) = Rz(y) Ry(p) Rx(r)."""
+    def rot(axis, a):
+        return gp.UnitQuaternion.from_axis_angle(axis, a).to_matrix()
+    q = gp.UnitQuaternion.from_euler_angles(0.1, 0.2, 0.3).to_matrix()
+    assert np.allclose(q, rot([0, 0, 1], 0.3) @ rot([0, 1, 0], 0.2) @ rot([1, 0, 0], 0.1), atol=1e-15)
+
+
+def test_workload_prefix_property(gp):
+    """Instance i gets the same parameters and state whatever the batch size."""
+    from gorilla_physics_amd import workloads
+    a, b = workloads.arms(n=5), workloads.arms(n=9)
+    assert np.array_equal(a["q"], b["q"][:35]) and np.array_equal(a["tau"], b["tau"][:35])
+    for i in range(5):
+        assert bytes(a["models"][i]._bodies) == bytes(b["models"][i]._bodies)
+
+
+def test_cube_workload_starts_above_plane(gp, oracle):
+    from gorilla_physics_amd import workloads
+    w = workloads.cubes(n=32)
+    for i in range(32):
+        d = oracle.contact_distances(w["models"][0], w["q"][7 * i:7 * i + 7])
+        assert 0.5 - 1e-12 <= d.min() <= 2.0 + 1e-12
+        assert abs(np.linalg.norm(w["q"][7 * i:7 * i + 4]) - 1) < 1e-12
+
+
+def test_mixed_workload_shape(gp):
+    from gorilla_physics_amd import workloads
+    w = workloads.mixed(n=400)
+    nb = [d.n_bodies for d in w["models"]]
+    trees = sum(1 for d in w["models"] if d.n_bodies == 13 and d._bodies[0].jtype == 3)
+    assert 0.15 < trees / 400 < 0.35
+    assert set(nb) - {13} <= set(range(1, 13))
+    assert sum(d.nq for d in w["models"]) == len(w["q"])
+    assert sum(d.nv for d in w["models"]) == len(w["v"]) == len(w["tau"])
