@@ -85,7 +85,8 @@ class gp_batch_opts(C.Structure):
         ("block_size", C.c_int32),
         ("lanes_per_mech", C.c_int32),
         ("use_graph", C.c_int32),
-        ("_reserved", C.c_int32 * 2),
+        ("separate_launches", C.c_int32),
+        ("_reserved", C.c_int32),
     ]
 
 

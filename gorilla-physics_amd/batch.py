@@ -36,10 +36,11 @@ class Model:
 
 
 def make_opts(precision=abi.GP_PREC_F64, kernel=abi.GP_KERNEL_AUTO, steps_per_launch=0, block_size=0,
-              lanes_per_mech=0, use_graph=0):
+              lanes_per_mech=0, use_graph=0, separate_launches=0):
     o = abi.gp_batch_opts()
     o.precision, o.kernel, o.steps_per_launch = precision, kernel, steps_per_launch
     o.block_size, o.lanes_per_mech, o.use_graph = block_size, lanes_per_mech, use_graph
+    o.separate_launches = separate_launches
     return o
 
 

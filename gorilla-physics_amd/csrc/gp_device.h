@@ -1,0 +1,123 @@
+// gp_device.h — device helpers shared by the plain and fast kernels:
+// contact force law, joint-angle sin/cos, the 6x6 Cholesky of a floating
+// joint's D and the floating-joint position update.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gp_internal.h"
+#include "gp_spatial.h"
+
+namespace gp {
+
+// Hunt-Crossley normal force + regularised Coulomb friction, force from the
+// half-space on the point (contact.rs:260-302). z^1.5 is z * sqrt(z): NaN for
+// z < 0 exactly like powf, and fmax(NaN, 0) = 0 like Rust's f64::max.
+template <class R>
+__device__ __forceinline__ v3<R> contact_force(R z, v3<R> n, v3<R> vel, R kA, R kB, R alpha, R mu) {
+  R z_dot = -dot(vel, n);
+  R zn = z * sqrt(z);
+  R k = kA * kB / (kA + kB);
+  R lam = R(1.5) * alpha * k;
+  R pi = fmax(lam * zn * z_dot + k * zn, R(0));
+  v3<R> f = pi * n;
+  v3<R> vt = vel + z_dot * n;
+  R vtn = sqrt(dot(vt, vt));
+  if (vtn != R(0)) {
+    R s = vtn / R(kSlipTol);
+    R mu_eff = s > R(1) ? mu : mu * s;
+    f = f - (mu_eff * pi / vtn) * vt;
+  }
+  return f;
+}
+
+// sin/cos of a joint angle held in FP64: exact sincos in FP64; for FP32 the
+// angle is first reduced to [-pi, pi] in FP64.
+__device__ __forceinline__ void sincos_q(double th, double* s, double* c) { sincos(th, s, c); }
+// FP32 path: quadrant reduction th = k pi/2 + r, |r| <= pi/4, in FP64 (two
+// FMAs with pi/2 split into a head and a tail, Cody & Waite 1980), then the
+// Taylor polynomials of sin and cos in FP32. Truncation after x^9 / x^10 is
+// below 2e-9 on |r| <= pi/4, under FP32 rounding.
+__device__ __forceinline__ void sincos_q(double th, float* s, float* c) {
+  const double pio2_hi = 1.5707963267948966192313216916398;  // fl(pi / 2)
+  const double pio2_lo = 6.123233995736766035868820147292e-17;  // pi / 2 - fl(pi / 2)
+  const double k = rint(th * 0.63661977236758134307553505349006);  // th * 2 / pi
+  const float r = float(fma(-k, pio2_lo, fma(-k, pio2_hi, th)));
+  const float x2 = r * r;
+  float sn = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, 1.0f / 362880.0f, -1.0f / 5040.0f), 1.0f / 120.0f), -1.0f / 6.0f), 1.0f);
+  sn *= r;
+  float cs = fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, fmaf(x2, -1.0f / 3628800.0f, 1.0f / 40320.0f), -1.0f / 720.0f),
+                                    1.0f / 24.0f), -0.5f), 1.0f);
+  const int q = int(k) & 3;  // quadrant: (s, c) -> (c, -s) per quarter turn
+  const float s1 = (q & 1) ? cs : sn;
+  const float c1 = (q & 1) ? sn : cs;
+  *s = (q & 2) ? -s1 : s1;
+  *c = ((q + 1) & 2) ? -c1 : c1;
+}
+
+// Cholesky of a symmetric positive-definite 6x6 (lower, packed by rows);
+// false if a pivot is not positive.
+__device__ __forceinline__ bool chol6(const double (&M)[6][6], double (&L)[21]) {
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double s = M[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= L[j * (j + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
+    ok = ok && (s > 0.0);
+    double d = sqrt(s);
+    L[j * (j + 1) / 2 + j] = d;
+    double inv = 1.0 / d;
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+      double t = M[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= L[i * (i + 1) / 2 + k] * L[j * (j + 1) / 2 + k];
+      L[i * (i + 1) / 2 + j] = t * inv;
+    }
+  }
+  return ok;
+}
+__device__ __forceinline__ void chol6_solve(const double (&L)[21], double (&x)[6]) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double s = x[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= L[i * (i + 1) / 2 + k] * x[k];
+    x[i] = s / L[i * (i + 1) / 2 + i];
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double s = x[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) s -= L[k * (k + 1) / 2 + i] * x[k];
+    x[i] = s / L[i * (i + 1) / 2 + i];
+  }
+}
+template <class R> __device__ __forceinline__ void to_dense(const ai6<R>& I, double (&M)[6][6]) {
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    sv<R> c = col(I, j);
+    M[0][j] = c.w.x; M[1][j] = c.w.y; M[2][j] = c.w.z; M[3][j] = c.v.x; M[4][j] = c.v.y; M[5][j] = c.v.z;
+  }
+}
+
+// Floating-joint position update q(k+1) = q(k) (+) dt v (integrators.rs:296-319,
+// util.rs:83-102): translation by the OLD rotation, quaternion derivative
+// 1/2 q (x) (0, w_body), then renormalisation (UnitQuaternion::from_quaternion).
+__device__ __forceinline__ void advance_floating(const double* qi, const double* vi, double dt, double* qo) {
+  double x = qi[0], y = qi[1], z = qi[2], w = qi[3];
+  double ox = vi[0], oy = vi[1], oz = vi[2];
+  double dw = (-x * ox - y * oy - z * oz) / 2.0;
+  double dx = (w * ox - z * oy + y * oz) / 2.0;
+  double dy = (z * ox + w * oy - x * oz) / 2.0;
+  double dz = (-y * ox + x * oy + w * oz) / 2.0;
+  v3<double> td = quat_rot(x, y, z, w, v3<double>{vi[3], vi[4], vi[5]});
+  double t0 = qi[4] + td.x * dt, t1 = qi[5] + td.y * dt, t2 = qi[6] + td.z * dt;
+  double nx = x + dx * dt, ny = y + dy * dt, nz = z + dz * dt, nw = w + dw * dt;
+  double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz + nw * nw);
+  qo[0] = nx * inv; qo[1] = ny * inv; qo[2] = nz * inv; qo[3] = nw * inv;
+  qo[4] = t0; qo[5] = t1; qo[6] = t2;
+}
+
+}  // namespace gp

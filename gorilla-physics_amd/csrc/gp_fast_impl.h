@@ -1,0 +1,450 @@
+// gp_fast_impl.h — templates of the register-resident fast kernels, shared by
+// the per-precision translation units (gp_fast_f32/f64.hip, gp_group_f32/f64.hip).
+#pragma once
+
+//
+//
+// One instance per lane (or per L lanes for contact-heavy free bodies), the
+// whole per-instance state in registers, n_steps semi-implicit Euler steps
+// (integrators.rs:276-319) per launch. The dynamics is Featherstone's
+// articulated-body algorithm (RBDA Table 7.1) in body frames whose z axis is
+// each 1-DOF joint's axis; it returns the joint accelerations the reference's
+// dynamics_continuous computes with CRBA + RNEA + LU (dynamics.rs:322-364).
+//
+// Arithmetic: R = double (FP64 everywhere) or R = float (FP32 kinematics,
+// bias forces, contact forces and both sweeps; the joint-space inertia D,
+// the bias u, the 6x6 floating-base solve and the q/v accumulation in FP64).
+//
+// Register budget: per 1-DOF body only (cos, sin, displacement), the bias
+// acceleration's four non-zero components, p^A, U, D and u are kept across
+// the sweeps; the Plucker transform E = (R_t R_z(q))^T and r are rebuilt from
+// the tree parameters when a sweep needs them.
+#include <hip/hip_runtime.h>
+
+#include "../../include/gorilla_amd.h"
+#include "gp_device.h"
+#include "gp_internal.h"
+#include "gp_spatial.h"
+
+namespace gp {
+
+template <int N_>
+struct TopoChain {
+  static constexpr int N = N_;
+  static constexpr __host__ __device__ int parent(int i) { return i - 1; }
+  static constexpr __host__ __device__ int dof(int) { return 1; }
+  static constexpr __host__ __device__ int qdim(int) { return 1; }
+};
+struct TopoFree {
+  static constexpr int N = 1;
+  static constexpr __host__ __device__ int parent(int) { return -1; }
+  static constexpr __host__ __device__ int dof(int) { return 6; }
+  static constexpr __host__ __device__ int qdim(int) { return 7; }
+};
+struct TopoQuad {  // floating base + 4 legs of 3 one-DOF joints
+  static constexpr int N = 13;
+  static constexpr __host__ __device__ int parent(int i) { return i == 0 ? -1 : ((i - 1) % 3 == 0 ? 0 : i - 1); }
+  static constexpr __host__ __device__ int dof(int i) { return i == 0 ? 6 : 1; }
+  static constexpr __host__ __device__ int qdim(int i) { return i == 0 ? 7 : 1; }
+};
+template <class T> constexpr __host__ __device__ int qoff_of(int i) {
+  int s = 0;
+  for (int k = 0; k < i; ++k) s += T::qdim(k);
+  return s;
+}
+template <class T> constexpr __host__ __device__ int voff_of(int i) {
+  int s = 0;
+  for (int k = 0; k < i; ++k) s += T::dof(k);
+  return s;
+}
+template <class T> constexpr int nq_of() { return qoff_of<T>(T::N); }
+template <class T> constexpr int nv_of() { return voff_of<T>(T::N); }
+
+template <class R> __device__ __forceinline__ const R* pick(const double* d, const float* f);
+template <> __device__ __forceinline__ const double* pick<double>(const double* d, const float*) { return d; }
+template <> __device__ __forceinline__ const float* pick<float>(const double*, const float* f) { return f; }
+
+template <class R, int L> __device__ __forceinline__ R lane_sum(R x) {
+#pragma unroll
+  for (int off = 1; off < L; off <<= 1) x += __shfl_xor(x, off, L);
+  return x;
+}
+
+// E = (R_t R_z)^T and r = p_t + d R_t e_z for an aligned 1-DOF joint
+template <class R>
+__device__ __forceinline__ void joint_frame(const m3<R>& Rt, v3<R> pt, R c, R s, R d, m3<R>& E, v3<R>& r) {
+  E = m3<R>{{c * Rt.a[0] + s * Rt.a[1], c * Rt.a[3] + s * Rt.a[4], c * Rt.a[6] + s * Rt.a[7],
+             -s * Rt.a[0] + c * Rt.a[1], -s * Rt.a[3] + c * Rt.a[4], -s * Rt.a[6] + c * Rt.a[7],
+             Rt.a[2], Rt.a[5], Rt.a[8]}};
+  r = pt + d * v3<R>{Rt.a[2], Rt.a[5], Rt.a[8]};
+}
+
+template <class R>
+struct Body1 {  // what a 1-DOF body keeps between the sweeps
+  R c, s, d;       // joint angle cos / sin (1, 0 for prismatic), displacement (0 for revolute)
+  R cw0, cw1;      // bias acceleration v x_m (S qd): angular x, y (z is 0)
+  R cv0, cv1;      // linear x, y (z is 0)
+  sv<R> pA;        // articulated bias force
+  sv<R> U;         // I^A S
+  double D, u;     // S^T I^A S and tau - S^T p^A
+  bool rev;
+};
+
+template <class Topo, class R>
+struct FastParams {
+  const R* bp;
+  int Bp, lane;
+  // uniform column base (scalar unit) + lane offset (one VGPR for all loads)
+  __device__ __forceinline__ R operator()(int f, int i) const {
+    const R* column = bp + (size_t)(f * Topo::N + i) * (size_t)Bp;
+    return column[lane];
+  }
+  __device__ __forceinline__ m3<R> Rt(int i) const {
+    m3<R> M;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) M.a[k] = (*this)(BF_RT + k, i);
+    return M;
+  }
+  __device__ __forceinline__ v3<R> pt(int i) const { return {(*this)(BF_PT, i), (*this)(BF_PT + 1, i), (*this)(BF_PT + 2, i)}; }
+  __device__ __forceinline__ ai6<R> inertia(int i) const {
+    s3<R> J{(*this)(BF_J, i), (*this)(BF_J + 1, i), (*this)(BF_J + 2, i), (*this)(BF_J + 3, i), (*this)(BF_J + 4, i),
+            (*this)(BF_J + 5, i)};
+    return rigid(J, v3<R>{(*this)(BF_C, i), (*this)(BF_C + 1, i), (*this)(BF_C + 2, i)}, (*this)(BF_M, i));
+  }
+};
+
+// Articulated-body dynamics of one instance with a compile-time topology.
+// ALLREV: every 1-DOF joint of the bucket is revolute (no per-lane select).
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
+__device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int sub, const double* q,
+                                              const double* v, const double* tau, double* vdot) {
+  constexpr int N = Topo::N;
+  const FastParams<Topo, R> P{pick<R>(a.bp, a.bpf), a.Bp, lane};
+  const int Bp = a.Bp;
+  Body1<R> b[N];
+  sv<R> vel[N];
+  // the floating root (if any) keeps its full transform
+  m3<R> E0;
+  v3<R> r0;
+  sv<R> cb0, pA0;
+  m3<double> Rw[CONTACTS ? N : 1];
+  v3<double> tw[CONTACTS ? N : 1];
+  bool ok = true;
+  // ---- pass 1: kinematics, velocities, bias terms (root to leaves)
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int pr = Topo::parent(i);
+    const int pp = pr < 0 ? 0 : pr;
+    m3<R> E;
+    v3<R> r;
+    sv<R> vj{{R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
+    m3<double> Rm_d;
+    v3<double> p_d;
+    if (Topo::dof(i) == 1) {
+      const bool rev = ALLREV || a.kind[i * Bp + lane] == K_REV;
+      b[i].rev = rev;
+      const double qi = q[qoff_of<Topo>(i)];
+      sincos_q(rev ? qi : 0.0, &b[i].s, &b[i].c);
+      b[i].d = rev ? R(0) : R(qi);
+      joint_frame(P.Rt(i), P.pt(i), b[i].c, b[i].s, b[i].d, E, r);
+      const R qd = R(v[voff_of<Topo>(i)]);
+      vj.w.z = rev ? qd : R(0);
+      vj.v.z = rev ? R(0) : qd;
+      if (CONTACTS) {
+        m3<double> Ed = cvt<double>(E);
+        Rm_d = m3<double>{{Ed.a[0], Ed.a[3], Ed.a[6], Ed.a[1], Ed.a[4], Ed.a[7], Ed.a[2], Ed.a[5], Ed.a[8]}};
+        p_d = cvt<double>(r);
+      }
+    } else {
+      const int o = qoff_of<Topo>(i), w = voff_of<Topo>(i);
+      m3<double> Rq = quat_mat(q[o], q[o + 1], q[o + 2], q[o + 3]);
+      v3<double> tq{q[o + 4], q[o + 5], q[o + 6]};
+      m3<double> Rtd;
+      v3<double> ptd;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Rtd.a[k] = a.bp[(size_t)((BF_RT + k) * N + i) * Bp + lane];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) (&ptd.x)[k] = a.bp[(size_t)((BF_PT + k) * N + i) * Bp + lane];
+      Rm_d = mul(Rtd, Rq);
+      p_d = ptd + mul(Rtd, tq);
+      const m3<R> Rm = cvt<R>(Rm_d);
+      E = m3<R>{{Rm.a[0], Rm.a[3], Rm.a[6], Rm.a[1], Rm.a[4], Rm.a[7], Rm.a[2], Rm.a[5], Rm.a[8]}};
+      r = cvt<R>(p_d);
+      vj.w = v3<R>{R(v[w]), R(v[w + 1]), R(v[w + 2])};
+      vj.v = v3<R>{R(v[w + 3]), R(v[w + 4]), R(v[w + 5])};
+      E0 = E;
+      r0 = r;
+    }
+    vel[i] = pr < 0 ? vj : xmotion(E, r, vel[pp]) + vj;
+    const sv<R>& V = vel[i];
+    ai6<R> I = P.inertia(i);
+    sv<R> pA = crf(V, mul(I, V));
+    if (Topo::dof(i) == 1) {
+      const R qd = b[i].rev ? vj.w.z : vj.v.z;
+      // v x_m (S qd) with S = e_z angular (revolute) or linear (prismatic)
+      b[i].cw0 = b[i].rev ? qd * V.w.y : R(0);
+      b[i].cw1 = b[i].rev ? -qd * V.w.x : R(0);
+      b[i].cv0 = qd * (b[i].rev ? V.v.y : V.w.y);
+      b[i].cv1 = -qd * (b[i].rev ? V.v.x : V.w.x);
+    } else {
+      cb0 = crm(V, vj);
+    }
+    if (CONTACTS) {
+      if (pr < 0) { Rw[i] = Rm_d; tw[i] = p_d; }
+      else { Rw[i] = mul(Rw[pp], Rm_d); tw[i] = tw[pp] + mul(Rw[pp], p_d); }
+      const m3<R> RwR = cvt<R>(Rw[i]);
+      sv<R> fe{{R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
+      const int c0 = a.cstart[i], c1 = a.cstart[i + 1];
+      for (int cc = c0 + sub; cc < c1; cc += L) {
+        auto cpar = [&](int fld) { return a.cp[((size_t)fld * a.nc + cc) * Bp + lane]; };
+        v3<double> pb{cpar(CF_P), cpar(CF_P + 1), cpar(CF_P + 2)};
+        const R kc = R(cpar(CF_K));
+        v3<double> pw = mul(Rw[i], pb) + tw[i];
+        v3<R> pbR = cvt<R>(pb);
+        v3<R> vw = mul(RwR, V.v + cross(V.w, pbR));
+        for (int h = 0; h < a.nh; ++h) {
+          auto hpar = [&](int fld) { return a.hp[((size_t)fld * a.nh + h) * Bp + lane]; };
+          v3<double> n{hpar(HF_NORMAL), hpar(HF_NORMAL + 1), hpar(HF_NORMAL + 2)};
+          double dist = dot(pw - v3<double>{hpar(HF_POINT), hpar(HF_POINT + 1), hpar(HF_POINT + 2)}, n);
+          if (dist <= kInsideMargin) {
+            v3<R> F = contact_force(R(-dist), cvt<R>(n), vw, kc, R(kHalfspaceK), R(hpar(HF_ALPHA)), R(hpar(HF_MU)));
+            v3<R> Fb = mulT(RwR, F);
+            fe.w = fe.w + cross(pbR, Fb);
+            fe.v = fe.v + Fb;
+          }
+        }
+      }
+      if (L > 1) {
+        fe.w.x = lane_sum<R, L>(fe.w.x); fe.w.y = lane_sum<R, L>(fe.w.y); fe.w.z = lane_sum<R, L>(fe.w.z);
+        fe.v.x = lane_sum<R, L>(fe.v.x); fe.v.y = lane_sum<R, L>(fe.v.y); fe.v.z = lane_sum<R, L>(fe.v.z);
+      }
+      pA = pA - fe;
+    }
+    if (Topo::dof(i) == 1) b[i].pA = pA; else pA0 = pA;
+  }
+  // ---- pass 2: articulated inertias (leaves to root)
+  ai6<R> acc[N];
+  bool has_acc[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) has_acc[i] = false;
+  double Lf[21], D6[6][6], u6[6];
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    const int pr = Topo::parent(i);
+    const int pp = pr < 0 ? 0 : pr;
+    ai6<R> IA = P.inertia(i);
+    if (has_acc[i]) add_to(IA, acc[i]);
+    if (Topo::dof(i) == 1) {
+      Body1<R>& B = b[i];
+      B.U = B.rev ? col(IA, 2) : col(IA, 5);
+      B.D = double(B.rev ? IA.A.zz : IA.C.zz);
+      double tq = tau[voff_of<Topo>(i)];
+      if (!B.rev) {
+        const R sk = P(BF_SK, i);
+        if (sk != R(0)) tq += -double(sk) * (q[qoff_of<Topo>(i)] - double(P(BF_SL, i)));
+      }
+      B.u = tq - double(B.rev ? B.pA.w.z : B.pA.v.z);
+      ok = ok && (B.D > 0.0);
+      if (pr >= 0) {
+        const R inv = R(1.0 / B.D);
+        const sv<R> uD{inv * B.U.w, inv * B.U.v};
+        const ai6<R> Ia = rank1_down(IA, B.U, uD);
+        const R uR = R(B.u);
+        const sv<R> cbv{{B.cw0, B.cw1, R(0)}, {B.cv0, B.cv1, R(0)}};
+        const sv<R> pa = B.pA + mul(Ia, cbv) + sv<R>{uR * uD.w, uR * uD.v};
+        m3<R> E;
+        v3<R> r;
+        joint_frame(P.Rt(i), P.pt(i), B.c, B.s, B.d, E, r);
+        const ai6<R> add = xtix(E, r, Ia);
+        if (has_acc[pp]) add_to(acc[pp], add);
+        else { acc[pp] = add; has_acc[pp] = true; }
+        if (Topo::dof(pp) == 1) b[pp].pA = b[pp].pA + xtforce(E, r, pa);
+        else pA0 = pA0 + xtforce(E, r, pa);
+      }
+    } else {
+      // floating joint (the root in every compiled topology): D = I^A (6x6)
+      to_dense(IA, D6);
+      const int w = voff_of<Topo>(i);
+      u6[0] = tau[w] - double(pA0.w.x); u6[1] = tau[w + 1] - double(pA0.w.y); u6[2] = tau[w + 2] - double(pA0.w.z);
+      u6[3] = tau[w + 3] - double(pA0.v.x); u6[4] = tau[w + 4] - double(pA0.v.y); u6[5] = tau[w + 5] - double(pA0.v.z);
+      ok = chol6(D6, Lf) && ok;
+    }
+  }
+  // ---- pass 3: accelerations (root to leaves)
+  sv<R> accel[N];
+  const sv<R> a0{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}};
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int pr = Topo::parent(i);
+    const int pp = pr < 0 ? 0 : pr;
+    if (Topo::dof(i) == 1) {
+      const Body1<R>& B = b[i];
+      m3<R> E;
+      v3<R> r;
+      joint_frame(P.Rt(i), P.pt(i), B.c, B.s, B.d, E, r);
+      sv<R> ap = xmotion(E, r, pr < 0 ? a0 : accel[pp]);
+      ap.w.x += B.cw0; ap.w.y += B.cw1; ap.v.x += B.cv0; ap.v.y += B.cv1;
+      const double qdd = (B.u - double(dot6(B.U, ap))) / B.D;
+      vdot[voff_of<Topo>(i)] = qdd;
+      if (B.rev) ap.w.z += R(qdd); else ap.v.z += R(qdd);
+      accel[i] = ap;
+    } else {
+      sv<R> ap = xmotion(E0, r0, pr < 0 ? a0 : accel[pp]) + cb0;
+      const int w = voff_of<Topo>(i);
+      const double apd[6] = {double(ap.w.x), double(ap.w.y), double(ap.w.z), double(ap.v.x), double(ap.v.y), double(ap.v.z)};
+      double x[6];
+#pragma unroll
+      for (int rr = 0; rr < 6; ++rr) {
+        double s = u6[rr];
+#pragma unroll
+        for (int cc = 0; cc < 6; ++cc) s -= D6[rr][cc] * apd[cc];
+        x[rr] = s;
+      }
+      chol6_solve(Lf, x);
+#pragma unroll
+      for (int t = 0; t < 6; ++t) vdot[w + t] = x[t];
+      accel[i] = ap + sv<R>{{R(x[0]), R(x[1]), R(x[2])}, {R(x[3]), R(x[4]), R(x[5])}};
+    }
+  }
+  return ok;
+}
+
+// n_steps semi-implicit Euler steps per launch with the state in registers;
+// q and v accumulate in FP64. With `active`, lane l runs min(n_steps,
+// active[l]) steps (serving: instances with different horizons).
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
+__device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, double dt, int n_steps) {
+  constexpr int NQ = nq_of<Topo>(), NV = nv_of<Topo>();
+  const int inst = tid / L;
+  const int sub = tid % L;
+  if (inst >= a.B) return;
+  const int Bp = a.Bp;
+  int my_steps = n_steps;
+  if (a.active) my_steps = min(my_steps, a.active[inst]);
+  double q[NQ], v[NV], tau[NV], vd[NV];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) q[k] = a.q[(size_t)k * Bp + inst];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    v[k] = a.v[(size_t)k * Bp + inst];
+    tau[k] = a.tau[(size_t)k * Bp + inst];
+    vd[k] = a.vdot[(size_t)k * Bp + inst];
+  }
+  bool ok = true;
+  for (int s = 0; s < my_steps; ++s) {
+    ok = fast_dynamics<Topo, R, L, CONTACTS, ALLREV>(a, inst, sub, q, v, tau, vd) && ok;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = v[k] + vd[k] * dt;
+#pragma unroll
+    for (int i = 0; i < Topo::N; ++i) {
+      if (Topo::dof(i) == 1) q[qoff_of<Topo>(i)] += v[voff_of<Topo>(i)] * dt;
+      else advance_floating(&q[qoff_of<Topo>(i)], &v[voff_of<Topo>(i)], dt, &q[qoff_of<Topo>(i)]);
+    }
+  }
+  if (sub != 0) return;
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    a.q[(size_t)k * Bp + inst] = q[k];
+    ok = ok && isfinite(q[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    a.v[(size_t)k * Bp + inst] = v[k];
+    a.vdot[(size_t)k * Bp + inst] = vd[k];
+    ok = ok && isfinite(v[k]) && isfinite(vd[k]);
+  }
+  if (!ok) a.fault[inst] = 1;
+}
+
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
+__global__ void __launch_bounds__(256) fast_sie_kernel(BucketArgs a, double dt, int n_steps) {
+  fast_sie_body<Topo, R, L, CONTACTS, ALLREV>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps);
+}
+
+enum { TOPO_FREE = 100, TOPO_QUAD = 101 };
+
+// ---------------------------------------------------------------- grouped launch
+// Every bucket of a batch in one launch: bucket b owns workgroups
+// [block_start[b], block_start[b + 1]) and each workgroup runs its bucket's
+// compile-time topology. Buckets of different sizes and depths then share the
+// chip in one dispatch instead of a series of small launches.
+template <class R>
+__device__ __forceinline__ void grouped_dispatch(const BucketArgs& a, int topo, int tid, double dt, int n) {
+  const bool ar = a.all_rev != 0;
+  switch (topo) {
+#define GP_CHAIN_CASE(K)                                                               \
+  case K:                                                                              \
+    if (ar) fast_sie_body<TopoChain<K>, R, 1, false, true>(a, tid, dt, n);             \
+    else fast_sie_body<TopoChain<K>, R, 1, false, false>(a, tid, dt, n);               \
+    break;
+    GP_CHAIN_CASE(1) GP_CHAIN_CASE(2) GP_CHAIN_CASE(3) GP_CHAIN_CASE(4) GP_CHAIN_CASE(5) GP_CHAIN_CASE(6)
+    GP_CHAIN_CASE(7) GP_CHAIN_CASE(8) GP_CHAIN_CASE(9) GP_CHAIN_CASE(10) GP_CHAIN_CASE(11) GP_CHAIN_CASE(12)
+#undef GP_CHAIN_CASE
+    case TOPO_QUAD:
+      if (ar) fast_sie_body<TopoQuad, R, 1, false, true>(a, tid, dt, n);
+      else fast_sie_body<TopoQuad, R, 1, false, false>(a, tid, dt, n);
+      break;
+    case TOPO_FREE:
+      switch (a.lanes) {
+        case 2: fast_sie_body<TopoFree, R, 2, true, false>(a, tid, dt, n); break;
+        case 4: fast_sie_body<TopoFree, R, 4, true, false>(a, tid, dt, n); break;
+        case 8: fast_sie_body<TopoFree, R, 8, true, false>(a, tid, dt, n); break;
+        default: fast_sie_body<TopoFree, R, 1, true, false>(a, tid, dt, n); break;
+      }
+      break;
+    default: break;
+  }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) grouped_sie_kernel(const GroupTable* __restrict__ t, double dt, int n_steps) {
+  int b = 0;
+  while (b + 1 < t->n && (int)blockIdx.x >= t->block_start[b + 1]) ++b;  // uniform per workgroup
+  const BucketArgs a = t->args[b];
+  const int tid = ((int)blockIdx.x - t->block_start[b]) * (int)blockDim.x + (int)threadIdx.x;
+  grouped_dispatch<R>(a, t->topo[b], tid, dt, n_steps);
+}
+
+
+template <class Topo, class R, int L, bool C, bool AR>
+static int launch_t(const BucketArgs& a, double dt, int n_steps, int block, hipStream_t s) {
+  long threads = (long)a.B * L;
+  dim3 blk(block), grid((unsigned)((threads + block - 1) / block));
+  hipLaunchKernelGGL((fast_sie_kernel<Topo, R, L, C, AR>), grid, blk, 0, s, a, dt, n_steps);
+  return (int)hipGetLastError();
+}
+
+template <class Topo, class R>
+static int launch_chain(const BucketArgs& a, double dt, int n, int block, hipStream_t s) {
+  if (a.all_rev) return launch_t<Topo, R, 1, false, true>(a, dt, n, block, s);
+  return launch_t<Topo, R, 1, false, false>(a, dt, n, block, s);
+}
+
+template <class R>
+static int launch_prec(int topo, const BucketArgs& a, double dt, int n, int block, hipStream_t s) {
+  switch (topo) {
+    case 1: return launch_chain<TopoChain<1>, R>(a, dt, n, block, s);
+    case 2: return launch_chain<TopoChain<2>, R>(a, dt, n, block, s);
+    case 3: return launch_chain<TopoChain<3>, R>(a, dt, n, block, s);
+    case 4: return launch_chain<TopoChain<4>, R>(a, dt, n, block, s);
+    case 5: return launch_chain<TopoChain<5>, R>(a, dt, n, block, s);
+    case 6: return launch_chain<TopoChain<6>, R>(a, dt, n, block, s);
+    case 7: return launch_chain<TopoChain<7>, R>(a, dt, n, block, s);
+    case 8: return launch_chain<TopoChain<8>, R>(a, dt, n, block, s);
+    case 9: return launch_chain<TopoChain<9>, R>(a, dt, n, block, s);
+    case 10: return launch_chain<TopoChain<10>, R>(a, dt, n, block, s);
+    case 11: return launch_chain<TopoChain<11>, R>(a, dt, n, block, s);
+    case 12: return launch_chain<TopoChain<12>, R>(a, dt, n, block, s);
+    case TOPO_QUAD: return launch_chain<TopoQuad, R>(a, dt, n, block, s);
+    case TOPO_FREE:
+      switch (a.lanes) {
+        case 2: return launch_t<TopoFree, R, 2, true, false>(a, dt, n, block, s);
+        case 4: return launch_t<TopoFree, R, 4, true, false>(a, dt, n, block, s);
+        case 8: return launch_t<TopoFree, R, 8, true, false>(a, dt, n, block, s);
+        default: return launch_t<TopoFree, R, 1, true, false>(a, dt, n, block, s);
+      }
+    default: return (int)hipErrorInvalidValue;
+  }
+}
+
+}  // namespace gp

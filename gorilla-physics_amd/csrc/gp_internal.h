@@ -60,7 +60,8 @@ struct BucketArgs {
   int B;      // instances
   int Bp;     // lane pitch
   int nb, nq, nv, nc, nh;
-  int lanes;  // lanes per instance (FAST kernels)
+  int lanes;    // lanes per instance (FAST kernels)
+  int all_rev;  // every 1-DOF joint of every lane is revolute
   double g;
   // topology (device, uniform over the bucket)
   const int* parent;  // [nb]
@@ -85,7 +86,19 @@ struct BucketArgs {
   double* scratch;     // plain kernel: [NPF][nb][Bp] + RK stages
 };
 
-// Host-side launchers (gp_kernels.hip). Each returns a hipError_t as int.
+// All buckets of a batch in one launch (fast topologies only): bucket b owns
+// workgroups [block_start[b], block_start[b + 1]).
+constexpr int kMaxGroup = 32;
+struct GroupTable {
+  int n;
+  int block_start[kMaxGroup + 1];
+  int topo[kMaxGroup];
+  BucketArgs args[kMaxGroup];
+};
+
+// Host-side launchers (gp_plain.hip, gp_fast.hip). Each returns a hipError_t as int.
+int launch_grouped_step(const GroupTable* d_table, int n_blocks, int precision, double dt, int n_steps, int block,
+                        void* stream);
 int launch_plain_step(const BucketArgs& a, double dt, int integrator, int dynamics_only, void* stream);
 // Fast kernels: topo_id from fast_topology_id(); precision GP_PREC_*.
 int fast_topology_id(int nb, const int* parent, const int* dofcls, int nc);

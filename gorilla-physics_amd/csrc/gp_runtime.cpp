@@ -268,6 +268,7 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models) {
   b.nb = m0->n; b.nq = m0->nq; b.nv = m0->nv; b.nc = m0->nc; b.nh = m0->nh; b.g = m0->g;
   b.parent = m0->parent; b.qoff = m0->qoff; b.voff = m0->voff; b.dofcls = m0->dofcls; b.cstart = m0->cstart;
   b.fast_topo = m0->fast_topo;
+  b.all_rev = true;
   b.B = (int)lane_models.size();
   b.Bp = (b.B + 63) / 64 * 64;
   const size_t Bp = b.Bp;
@@ -277,6 +278,7 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models) {
     const gp_model* m = lane_models[l];
     for (int i = 0; i < b.nb; ++i) {
       kind[i * Bp + l] = m->kind[i];
+      if (m->kind[i] == K_PRI) b.all_rev = false;
       for (int f = 0; f < NBF; ++f) bp[((size_t)f * b.nb + i) * Bp + l] = m->bp[(size_t)i * NBF + f];
     }
     for (int c = 0; c < b.nc; ++c)
@@ -320,7 +322,7 @@ int bucket_ensure_scratch(Bucket& b) {
 BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
   BucketArgs a{};
   a.B = b.B; a.Bp = b.Bp; a.nb = b.nb; a.nq = b.nq; a.nv = b.nv; a.nc = b.nc; a.nh = b.nh;
-  a.lanes = lanes; a.g = b.g;
+  a.lanes = lanes; a.g = b.g; a.all_rev = b.all_rev ? 1 : 0;
   a.parent = b.d_parent; a.qoff = b.d_qoff; a.voff = b.d_voff; a.cstart = b.d_cstart;
   a.kind = b.d_kind; a.bp = b.d_bp; a.bpf = b.d_bpf; a.cp = b.d_cp; a.cpf = b.d_cpf; a.hp = b.d_hp; a.hpf = b.d_hpf;
   a.q = b.d_q; a.v = b.d_v; a.tau = tau ? tau : b.d_tau; a.vdot = b.d_vdot; a.fault = b.d_fault;
@@ -333,15 +335,73 @@ BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
 using namespace gp;
 
 // ------------------------------------------------------------------ batch
+static bool bucket_is_fast(const gp_batch_opts& o, const Bucket& b, int integrator) {
+  return o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0 && integrator == GP_SEMI_IMPLICIT_EULER;
+}
+static int bucket_lanes(const gp_batch_opts& o, const Bucket& b) {
+  return b.fast_topo == 100 && o.lanes_per_mech > 0 ? o.lanes_per_mech : 1;
+}
+static int group_block_size(const gp_batch_opts& o) { return o.block_size > 0 ? o.block_size : 128; }
+
+// Number of FAST buckets that one grouped launch would cover (0 = no grouping).
+static int grouped_count(const gp_batch* bt, int integrator) {
+  const gp_batch_opts& o = bt->opts;
+  if (o.separate_launches) return 0;
+  int n = 0;
+  for (const Bucket& b : bt->buckets) n += bucket_is_fast(o, b, integrator) ? 1 : 0;
+  return (n >= 2 && n <= kMaxGroup) ? n : 0;
+}
+
+// Build (or reuse) the device table of the grouped launch. Synchronous: it
+// must run before any graph capture.
+static int ensure_group(gp_batch* bt, int integrator) {
+  const gp_batch_opts& o = bt->opts;
+  const int block = group_block_size(o);
+  if (bt->group_valid && bt->group_block == block) return GP_OK;
+  GroupTable t{};
+  int blocks = 0;
+  for (const Bucket& b : bt->buckets) {
+    if (!bucket_is_fast(o, b, integrator)) continue;
+    const int lanes = bucket_lanes(o, b);
+    t.args[t.n] = bucket_args(b, lanes, nullptr);
+    t.topo[t.n] = b.fast_topo;
+    t.block_start[t.n] = blocks;
+    blocks += (int)(((int64_t)b.B * lanes + block - 1) / block);
+    ++t.n;
+  }
+  t.block_start[t.n] = blocks;
+  if (!bt->d_group) {
+    hipError_t e = hipMalloc((void**)&bt->d_group, sizeof(GroupTable));
+    if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(group table): %s", hipGetErrorString(e));
+  }
+  HIP_TRY(hipMemcpy(bt->d_group, &t, sizeof t, hipMemcpyHostToDevice));
+  bt->group_blocks = blocks;
+  bt->group_block = block;
+  bt->group_valid = true;
+  return GP_OK;
+}
+
 static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps, hipStream_t s, int64_t* launches) {
   const gp_batch_opts& o = bt->opts;
+  const bool grouped = grouped_count(bt, integrator) > 0;
+  if (grouped) {
+    if (!bt->group_valid) return fail(GP_ERR_INVALID, "group table not prepared");
+    int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
+    for (int done = 0; done < n_steps; done += spl) {
+      int n = std::min(spl, n_steps - done);
+      int e = launch_grouped_step(bt->d_group, bt->group_blocks, o.precision, dt, n, bt->group_block, s);
+      if (e) return fail(GP_ERR_DEVICE, "grouped kernel launch: %s", hipGetErrorString((hipError_t)e));
+      ++*launches;
+    }
+  }
   for (Bucket& b : bt->buckets) {
-    bool fast = o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0 && integrator == GP_SEMI_IMPLICIT_EULER;
+    bool fast = bucket_is_fast(o, b, integrator);
     if (o.kernel == GP_KERNEL_FAST && !fast)
       return fail(GP_ERR_UNSUPPORTED, "no FAST kernel for this bucket's topology / integrator");
+    if (fast && grouped) continue;
     if (fast) {
       int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
-      int lanes = b.fast_topo == 100 && o.lanes_per_mech > 0 ? o.lanes_per_mech : 1;
+      int lanes = bucket_lanes(o, b);
       BucketArgs a = bucket_args(b, lanes, nullptr);
       for (int done = 0; done < n_steps; done += spl) {
         int n = std::min(spl, n_steps - done);
@@ -481,6 +541,7 @@ void gp_batch_destroy(gp_batch* bt) {
   if (!bt) return;
   (void)hipDeviceSynchronize();
   for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
+  if (bt->d_group) (void)hipFree(bt->d_group);
   if (bt->stream) (void)hipStreamDestroy(bt->stream);
   if (bt->ev0) (void)hipEventDestroy(bt->ev0);
   if (bt->ev1) (void)hipEventDestroy(bt->ev1);
@@ -507,9 +568,12 @@ int gp_batch_set_opts(gp_batch* bt, const gp_batch_opts* o) {
   if (!bt || !o) return fail(GP_ERR_INVALID, "null argument");
   int st = validate_opts(*o);
   if (st) return st;
+  HIP_TRY(hipStreamSynchronize(bt->stream));
   bt->opts = *o;
   for (auto& kv : bt->graphs) (void)hipGraphExecDestroy(kv.second);
   bt->graphs.clear();
+  bt->graph_launches.clear();
+  bt->group_valid = false;
   return GP_OK;
 }
 
@@ -580,6 +644,10 @@ int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* 
   if (n_steps < 0) return fail(GP_ERR_INVALID, "n_steps < 0");
   hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
   int64_t launches = 0;
+  if (grouped_count(bt, integrator) > 0) {
+    int st = ensure_group(bt, integrator);
+    if (st) return st;
+  }
   HIP_TRY(hipEventRecord(bt->ev0, s));
   bool graph = bt->opts.use_graph != 0 && n_steps > 0;
   if (graph) {

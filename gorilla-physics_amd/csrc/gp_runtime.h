@@ -30,6 +30,7 @@ namespace gp {
 
 struct Bucket {
   int nb = 0, nq = 0, nv = 0, nc = 0, nh = 0, B = 0, Bp = 0, fast_topo = -1;
+  bool all_rev = false;  // every 1-DOF joint of every lane is revolute
   double g = 9.81;
   std::vector<int> parent, qoff, voff, dofcls, cstart;
   std::vector<int64_t> inst;  // lane -> instance index
@@ -48,6 +49,7 @@ struct Bucket {
     if (this != &o) {
       release();
       nb = o.nb; nq = o.nq; nv = o.nv; nc = o.nc; nh = o.nh; B = o.B; Bp = o.Bp; fast_topo = o.fast_topo; g = o.g;
+      all_rev = o.all_rev;
       parent = std::move(o.parent); qoff = std::move(o.qoff); voff = std::move(o.voff); dofcls = std::move(o.dofcls);
       cstart = std::move(o.cstart); inst = std::move(o.inst);
       d_parent = o.d_parent; d_qoff = o.d_qoff; d_voff = o.d_voff; d_cstart = o.d_cstart; d_kind = o.d_kind;
@@ -84,4 +86,8 @@ struct gp_batch {
   int64_t last_launches = 0;
   std::map<std::tuple<double, int, int>, hipGraphExec_t> graphs;
   std::map<std::tuple<double, int, int>, int64_t> graph_launches;
+  // grouped launch over the FAST buckets (built lazily, rebuilt on set_opts)
+  gp::GroupTable* d_group = nullptr;
+  int group_blocks = 0, group_block = 0;
+  bool group_valid = false;
 };

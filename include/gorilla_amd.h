@@ -129,7 +129,9 @@ typedef struct gp_batch_opts {
   int32_t block_size;        /* threads per workgroup (0 = 256)               */
   int32_t lanes_per_mech;    /* lanes cooperating on one instance (0 = 1)     */
   int32_t use_graph;         /* capture multi-launch rollouts in a hipGraph   */
-  int32_t _reserved[2];
+  int32_t separate_launches; /* 1: one launch per topology bucket instead of
+                                one grouped launch over all FAST buckets      */
+  int32_t _reserved;
 } gp_batch_opts;
 
 typedef struct gp_model gp_model;

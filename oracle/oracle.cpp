@@ -85,11 +85,12 @@ int oracle_step(const gp_model_desc* d, int integrator, int method, const double
 
 // In-place rollout of n_inst instances (instance i uses descs[i]) over n_steps
 // steps with held torques, on n_threads host threads (one instance per thread
-// at a time). State arrays are flattened instance-major. Returns the number of
+// at a time). State arrays are flattened instance-major; vdot_last (may be
+// null) receives the accelerations of the last step. Returns the number of
 // instances whose solve failed.
 int oracle_rollout(const gp_model_desc* const* descs, int64_t n_inst, double* q, double* v,
                    const double* tau, double dt, int integrator, int method, int64_t n_steps,
-                   int n_threads) {
+                   int n_threads, double* vdot_last) {
   struct Prep { Model<double> M; AbaModel<double> A; };
   std::map<const gp_model_desc*, int> idx;
   std::vector<Prep> preps;
@@ -123,7 +124,8 @@ int oracle_rollout(const gp_model_desc* const* descs, int64_t n_inst, double* q,
         return dyn_any(MM, &P.A, method, qq, vv, ta, out);
       };
       for (int64_t s = 0; s < n_steps; ++s) {
-        if (step_integrate(P.M, integrator, qi, vi, ti, dt, qn.data(), vn.data(), (double*)nullptr, dyn)) {
+        double* ad = (vdot_last && s == n_steps - 1) ? vdot_last + vo[i] : nullptr;
+        if (step_integrate(P.M, integrator, qi, vi, ti, dt, qn.data(), vn.data(), ad, dyn)) {
           failures.fetch_add(1);
           break;
         }

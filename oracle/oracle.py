@@ -31,7 +31,7 @@ class Oracle:
         lib.oracle_mass_matrix.argtypes = [P, _D, _D, _D, _D]
         lib.oracle_step.argtypes = [P, C.c_int, C.c_int, _D, _D, _D, C.c_double, _D, _D, _D]
         lib.oracle_rollout.argtypes = [C.POINTER(P), C.c_int64, _D, _D, _D, C.c_double, C.c_int, C.c_int,
-                                       C.c_int64, C.c_int]
+                                       C.c_int64, C.c_int, _D]
         lib.oracle_simulate.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_int, _D, _D, _D, _D, _D,
                                         C.c_int64, C.POINTER(C.c_int64)]
         lib.oracle_energy.argtypes = [P, _D, _D, _D, _D]
@@ -76,15 +76,17 @@ class Oracle:
             raise RuntimeError("oracle: singular mass matrix")
         return qo, vo, ao
 
-    def rollout(self, descs, q, v, tau, dt, steps, integrator=0, method=CRBA, threads=1):
-        """descs: one ModelDesc per instance; flattened state; returns new (q, v)."""
+    def rollout(self, descs, q, v, tau, dt, steps, integrator=0, method=CRBA, threads=1, want_vdot=False):
+        """descs: one ModelDesc per instance; flattened state; returns new
+        (q, v, failures) or (q, v, vdot_of_last_step, failures)."""
         n = len(descs)
         arr = (C.c_void_p * n)(*[C.cast(d.ptr(), C.c_void_p).value for d in descs])
         q = np.array(q, dtype=float, copy=True); v = np.array(v, dtype=float, copy=True)
         t = np.ascontiguousarray(tau, float) if tau is not None else None
+        a = np.zeros(v.size) if want_vdot else None
         fails = self.lib.oracle_rollout(arr, n, _dp(q), _dp(v), _dp(t) if t is not None else None, dt, integrator,
-                                        method, int(steps), int(threads))
-        return q, v, fails
+                                        method, int(steps), int(threads), _dp(a) if want_vdot else None)
+        return (q, v, a, fails) if want_vdot else (q, v, fails)
 
     def simulate(self, d, final_time, dt, q0, v0, tau=None, integrator=0, method=CRBA):
         nq, nv = self.dims(d)
