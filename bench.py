@@ -81,6 +81,80 @@ def run_variant(w, models, steps, warmup, **opts):
                 faults=faults, instances=n)
 
 
+def serving_requests(n, seed=0):
+    """Requests mixing the mixed-workload mechanisms (chains and quadrupeds,
+    dt 1e-3) and free cubes on a plane (dt 1e-4), horizons U{100..2000}."""
+    from gorilla_physics_amd import batch, workloads
+    n_cubes = n // 4
+    mix = workloads.mixed(n=n - n_cubes, seed=seed)
+    cub = workloads.cubes(n=max(1, n_cubes), seed=seed)
+    cube_model = batch.Model(cub["models"][0])
+    rng = np.random.default_rng(seed + 17)
+    reqs, qo, vo = [], 0, 0
+    for d in mix["models"]:
+        reqs.append(dict(model=batch.Model(d), q=mix["q"][qo:qo + d.nq], v=mix["v"][vo:vo + d.nv],
+                         tau=mix["tau"][vo:vo + d.nv], dt=1e-3))
+        qo += d.nq
+        vo += d.nv
+    for i in range(n_cubes):
+        reqs.append(dict(model=cube_model, q=cub["q"][7 * i:7 * i + 7], v=cub["v"][6 * i:6 * i + 6],
+                         tau=np.zeros(6), dt=1e-4))
+    order = rng.permutation(len(reqs))
+    reqs = [reqs[i] for i in order]
+    for r in reqs:
+        r["horizon"] = int(rng.integers(100, 2001))
+    return reqs
+
+
+def serve_run(reqs, rate, chunk=50, slots=4096, seed=1):
+    """Poisson arrivals at `rate` requests/s into one server; returns
+    throughput and latency percentiles."""
+    from gorilla_physics_amd import abi, serve
+    s = serve.Server(precision=abi.GP_PREC_MIXED, chunk_steps=chunk, slots_per_pool=slots)
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(reqs))
+    t0 = s.clock() + 0.02
+    arrivals = t0 + np.cumsum(gaps)
+    ids = [s.submit(r["model"], r["q"], r["v"], r["tau"], r["horizon"], r["dt"], float(a))
+           for r, a in zip(reqs, arrivals)]
+    s.run()
+    t_end = s.clock()
+    lat = np.array([s.result(i)[2] for i in ids])
+    st = s.stats()
+    steps = float(sum(r["horizon"] for r in reqs))
+    span = t_end - arrivals[0]
+    return dict(rate=rate, requests=len(reqs), env_steps_per_s=steps / span, requests_per_s=len(reqs) / span,
+                p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                rounds=int(st["rounds"]), pools=int(st["pools"]), gpu_busy_s=st["busy_s"], makespan_s=span)
+
+
+def serve_baseline_plain(reqs, rate, seed=1):
+    """Plain FP64 per-request launches: each request alone in its own batch,
+    one plain-kernel launch per step, served first-in first-out. Service times
+    are measured; the FIFO completion times replay the same Poisson arrivals."""
+    from gorilla_physics_amd import abi, batch
+    service = []
+    for r in reqs:
+        b = batch.Batch([r["model"]], [1], precision=abi.GP_PREC_F64, kernel=abi.GP_KERNEL_PLAIN)
+        b.set_state(r["q"], r["v"])
+        b.set_tau(r["tau"])
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(r["dt"], 0, r["horizon"])
+        b.get_state()
+        service.append(time.perf_counter() - t0)
+        del b
+    gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(reqs))
+    arrivals = np.cumsum(gaps)
+    done, lat = 0.0, []
+    for a, sv in zip(arrivals, service):
+        done = max(a, done) + sv
+        lat.append(done - a)
+    steps = float(sum(r["horizon"] for r in reqs))
+    return dict(rate=rate, requests=len(reqs), env_steps_per_s=steps / (done - arrivals[0]),
+                p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
+                mean_service_ms=float(np.mean(service) * 1e3))
+
+
 def cpu_baseline(w, budget_s=12.0):
     """The oracle (CRBA + RNEA + LU, the reference's algorithm) over a bounded
     sample of the same workload, one instance per thread at a time."""
@@ -98,8 +172,8 @@ def cpu_baseline(w, budget_s=12.0):
     qo = np.concatenate([[0], np.cumsum(nq)])
     vo = np.concatenate([[0], np.cumsum(nv)])
     # calibrate on a small slice, then size the sample to about budget_s
-    n = min(n_all, 4 * cores)
-    steps = 20
+    n = min(n_all, 64 * cores)
+    steps = 50
     t0 = time.perf_counter()
     o.rollout(descs[:n], w["q"][:qo[n]], w["v"][:vo[n]], w["tau"][:vo[n]], w["dt"], steps, threads=cores)
     dt_cal = max(time.perf_counter() - t0, 1e-6)
@@ -128,6 +202,11 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rates", default="20000")
+    ap.add_argument("--serve-chunk", type=int, default=50)
+    ap.add_argument("--serve-baseline", type=int, default=100, help="requests replayed with plain FP64 launches")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: instances are independent, so several "
@@ -135,6 +214,25 @@ def main():
         sys.exit(2)
     load_package()
     from gorilla_physics_amd import abi, batch
+    serving = None
+    if not args.no_serve:
+        reqs = serving_requests(args.serve_requests)
+        rates = [float(x) for x in args.serve_rates.split(",")]
+        sweep = []
+        for rate in rates:
+            res = serve_run(reqs, rate, chunk=args.serve_chunk)
+            log("serve", res)
+            sweep.append(res)
+        base = None
+        if args.serve_baseline:
+            base = serve_baseline_plain(reqs[:args.serve_baseline], rates[0])
+            log("serve baseline (plain FP64 per request)", base)
+        serving = dict(sweep=sweep, plain_fp64_per_request=base,
+                       workload="Poisson arrivals; 75% chains 1-12 DOF / floating quadrupeds (dt 1e-3), "
+                                "25% free cubes on a plane (dt 1e-4); horizons U{100..2000}; mixed precision")
+        if args.workload == "serve":
+            print(json.dumps(dict(metric="serving: env-steps/s and p50/p99 rollout latency", serving=serving)))
+            return
     w = make_workload(args.workload, args.instances)
     models = [batch.Model(d) for d in w["models"]]
     log(f"workload {args.workload}: {sum(w['counts'])} instances")
@@ -196,6 +294,11 @@ def main():
         "cpu_baseline": cpu,
         "faults": mixed["faults"],
     }
+    if serving:
+        s0 = serving["sweep"][0]
+        line["latency_ms"] = {"p50": s0["p50_ms"], "p99": s0["p99_ms"], "rate_req_per_s": s0["rate"],
+                              "requests": s0["requests"]}
+        line["serving"] = serving
     line.update(extra)
     print(json.dumps(line))
 

@@ -94,7 +94,7 @@ class gp_server_opts(C.Structure):
     _fields_ = [
         ("precision", C.c_int32),
         ("chunk_steps", C.c_int32),
-        ("max_slots_per_bucket", C.c_int32),
+        ("slots_per_pool", C.c_int32),
         ("block_size", C.c_int32),
         ("mem_budget_bytes", C.c_int64),
         ("_reserved", C.c_int32 * 4),
@@ -134,6 +134,16 @@ SIGNATURES = {
     "gp_batch_faults": (C.c_int, [_P, C.POINTER(C.c_uint8)]),
     "gp_batch_clear_faults": (C.c_int, [_P]),
     "gp_batch_last_timing": (C.c_int, [_P, _D, C.POINTER(C.c_int64)]),
+    "gp_server_create": (C.c_int, [C.POINTER(gp_server_opts), C.POINTER(_P)]),
+    "gp_server_destroy": (None, [_P]),
+    "gp_server_clock": (C.c_int, [_P, _D]),
+    "gp_server_submit": (C.c_int, [_P, _P, _D, _D, _D, C.c_int64, C.c_double, C.c_double,
+                                   C.POINTER(C.c_int64)]),
+    "gp_server_run": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    "gp_server_pending": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "gp_server_completed": (C.c_int, [_P, C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]),
+    "gp_server_result": (C.c_int, [_P, C.c_int64, _D, _D, _D, C.POINTER(C.c_int32)]),
+    "gp_server_stats": (C.c_int, [_P, _D]),
 }
 
 _lib = None

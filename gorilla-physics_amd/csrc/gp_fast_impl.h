@@ -2,8 +2,6 @@
 // the per-precision translation units (gp_fast_f32/f64.hip, gp_group_f32/f64.hip).
 #pragma once
 
-//
-//
 // One instance per lane (or per L lanes for contact-heavy free bodies), the
 // whole per-instance state in registers, n_steps semi-implicit Euler steps
 // (integrators.rs:276-319) per launch. The dynamics is Featherstone's
@@ -320,7 +318,14 @@ __device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, doub
   if (inst >= a.B) return;
   const int Bp = a.Bp;
   int my_steps = n_steps;
-  if (a.active) my_steps = min(my_steps, a.active[inst]);
+  if (a.active) {
+    // serving: an idle slot does nothing; a busy one runs what is left of its
+    // horizon in this chunk and counts it down
+    const int left = a.active[inst];
+    if (left <= 0) return;
+    my_steps = min(my_steps, left);
+    if (sub == 0) a.active[inst] = left - my_steps;
+  }
   double q[NQ], v[NV], tau[NV], vd[NV];
 #pragma unroll
   for (int k = 0; k < NQ; ++k) q[k] = a.q[(size_t)k * Bp + inst];
@@ -402,7 +407,7 @@ __global__ void __launch_bounds__(256) grouped_sie_kernel(const GroupTable* __re
   while (b + 1 < t->n && (int)blockIdx.x >= t->block_start[b + 1]) ++b;  // uniform per workgroup
   const BucketArgs a = t->args[b];
   const int tid = ((int)blockIdx.x - t->block_start[b]) * (int)blockDim.x + (int)threadIdx.x;
-  grouped_dispatch<R>(a, t->topo[b], tid, dt, n_steps);
+  grouped_dispatch<R>(a, t->topo[b], tid, a.dt > 0.0 ? a.dt : dt, n_steps);
 }
 
 

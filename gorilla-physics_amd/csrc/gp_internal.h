@@ -63,6 +63,7 @@ struct BucketArgs {
   int lanes;    // lanes per instance (FAST kernels)
   int all_rev;  // every 1-DOF joint of every lane is revolute
   double g;
+  double dt;    // step of this bucket (serving pools); 0 = the launch's dt
   // topology (device, uniform over the bucket)
   const int* parent;  // [nb]
   const int* qoff;    // [nb]
@@ -82,7 +83,8 @@ struct BucketArgs {
   const double* tau;   // [nv][Bp]
   double* vdot;        // [nv][Bp] acceleration of the last step
   int* fault;          // [Bp]
-  const int* active;   // [Bp] or null: per-lane remaining steps (serving)
+  int* active;         // [Bp] or null: per-lane remaining steps (serving);
+                       // the FAST kernels count it down by the steps they run
   double* scratch;     // plain kernel: [NPF][nb][Bp] + RK stages
 };
 

@@ -151,6 +151,7 @@ int model_create(const gp_model_desc* d, gp_model** out) {
     int jt = b.jtype;
     m->kind[i] = jt == GP_JOINT_REVOLUTE ? K_REV : jt == GP_JOINT_PRISMATIC ? K_PRI : jt == GP_JOINT_FLOATING ? K_FLOAT : K_FIXED;
     m->dofcls[i] = jt == GP_JOINT_FLOATING ? 6 : jt == GP_JOINT_FIXED ? 0 : 1;
+    if (jt == GP_JOINT_PRISMATIC) m->all_rev = false;
     m->qoff[i] = m->nq;
     m->voff[i] = m->nv;
     m->nq += jt == GP_JOINT_FLOATING ? 7 : m->dofcls[i];

@@ -24,6 +24,7 @@ struct gp_model {
   std::vector<int> contact_order;  // sorted slot -> index in the description
   uint64_t sig = 0;
   int fast_topo = -1;
+  bool all_rev = true;  // no prismatic joint
 };
 
 namespace gp {

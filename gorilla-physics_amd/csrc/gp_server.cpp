@@ -1,0 +1,541 @@
+// gp_server.cpp — continuous-batching request loop over HBM-resident pools.
+//
+// Round structure (gp_server_run):
+//   1. requests whose arrival time has passed join the ready queue;
+//   2. ready requests are admitted into free slots of a pool keyed by
+//      (topology signature, dt); a pool is created only while the device
+//      memory budget allows it (memory-based admission); all admissions of a
+//      round go to the device in one pinned host->device copy and one admit
+//      launch per pool;
+//   3. one grouped launch advances every busy slot of every pool by up to
+//      chunk_steps steps (a slot runs what is left of its horizon; the kernel
+//      counts its remaining steps down, idle slots exit at once);
+//   4. slots whose horizon ended are packed by one gather launch per pool and
+//      copied back in one device->host copy; their requests complete with
+//      latency = completion - arrival on the server clock.
+// A request's arithmetic does not depend on its slot, its pool mates or the
+// round it was admitted in: the same kernel body runs it whatever the mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gorilla_amd.h"
+#include "gp_internal.h"
+#include "gp_runtime.h"
+#include "gp_serve.h"
+
+using namespace gp;
+
+namespace {
+
+struct Request {
+  const gp_model* model = nullptr;
+  std::vector<double> q0, v0, tau;
+  int64_t horizon = 0;
+  double dt = 0, arrival = 0, done_t = 0;
+  int pool = -1, slot = -1;
+  int64_t remaining = 0;
+  std::vector<double> q, v;
+  int fault = 0;
+  bool finished = false;
+};
+
+struct Pool {
+  uint64_t sig = 0;
+  double dt = 0;
+  bool all_rev = false;
+  Bucket b;
+  std::vector<int64_t> slot_req;  // request id per slot, -1 = free
+  std::vector<int> free_slots;
+  int busy = 0;
+  size_t bytes = 0;
+};
+
+template <class T>
+size_t round_up(size_t x) {
+  return (x + sizeof(T) - 1) / sizeof(T) * sizeof(T);
+}
+
+}  // namespace
+
+struct gp_server {
+  gp_server_opts o{};
+  std::vector<std::unique_ptr<Pool>> pools;
+  std::multimap<double, int64_t> future;  // arrival time -> request id
+  std::deque<int64_t> ready;
+  std::unordered_map<int64_t, Request> req;
+  std::vector<int64_t> done;
+  int64_t next_id = 1;
+  std::chrono::steady_clock::time_point t0;
+  hipStream_t stream = nullptr;
+  GroupTable* d_group = nullptr;
+  bool group_dirty = true;
+  int group_blocks = 0;
+  char* d_stage = nullptr;
+  size_t d_stage_cap = 0;
+  char* h_stage = nullptr;
+  size_t h_stage_cap = 0;
+  double rounds = 0, env_steps = 0, busy_s = 0, admitted = 0, completed = 0;
+  size_t mem_used = 0, budget = 0;
+  int64_t running = 0;
+
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
+
+namespace {
+
+int ensure_stage(gp_server* s, size_t bytes) {
+  if (bytes > s->d_stage_cap) {
+    if (s->d_stage) (void)hipFree(s->d_stage);
+    s->d_stage = nullptr;
+    size_t cap = std::max(bytes, 2 * s->d_stage_cap);
+    hipError_t e = hipMalloc((void**)&s->d_stage, cap);
+    if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(staging): %s", hipGetErrorString(e));
+    s->d_stage_cap = cap;
+  }
+  if (bytes > s->h_stage_cap) {
+    if (s->h_stage) (void)hipHostFree(s->h_stage);
+    s->h_stage = nullptr;
+    size_t cap = std::max(bytes, 2 * s->h_stage_cap);
+    hipError_t e = hipHostMalloc((void**)&s->h_stage, cap, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipHostMalloc(staging): %s", hipGetErrorString(e));
+    s->h_stage_cap = cap;
+  }
+  return GP_OK;
+}
+
+PoolPtrs pool_ptrs(Pool& p) {
+  PoolPtrs x{};
+  Bucket& b = p.b;
+  x.Bp = b.Bp; x.nb = b.nb; x.nq = b.nq; x.nv = b.nv; x.nc = b.nc; x.nh = b.nh;
+  x.kind = b.d_kind; x.bp = b.d_bp; x.bpf = b.d_bpf; x.cp = b.d_cp; x.cpf = b.d_cpf; x.hp = b.d_hp; x.hpf = b.d_hpf;
+  x.q = b.d_q; x.v = b.d_v; x.tau = b.d_tau; x.vdot = b.d_vdot; x.fault = b.d_fault; x.active = b.d_active;
+  return x;
+}
+
+size_t pool_bytes(const gp_model* m, int Bp) {
+  const size_t per_lane = (size_t)m->n * (NBF * 12 + 4) + (size_t)m->nc * NCF * 12 + (size_t)m->nh * NHF * 12 +
+                          (size_t)std::max(1, m->nq) * 8 + 4 * (size_t)std::max(1, m->nv) * 8 + 8;
+  return per_lane * Bp;
+}
+
+// A pool with a free slot for (sig, dt), creating one if the budget allows.
+int find_pool(gp_server* s, const Request& r, int* out) {
+  const gp_model* m = r.model;
+  for (size_t i = 0; i < s->pools.size(); ++i) {
+    Pool& p = *s->pools[i];
+    // the all-revolute kernel variant is part of the key: a pool built for
+    // revolute-only chains must not take a prismatic joint
+    if (p.sig == m->sig && p.dt == r.dt && p.all_rev == m->all_rev && !p.free_slots.empty()) {
+      *out = (int)i;
+      return GP_OK;
+    }
+  }
+  const int cap = s->o.slots_per_pool;
+  const int Bp = (cap + 63) / 64 * 64;
+  const size_t bytes = pool_bytes(m, Bp);
+  if (s->mem_used + bytes > s->budget) {
+    // release idle pools (no busy slot) to make room, oldest first
+    for (auto& pp : s->pools) {
+      if (s->mem_used + bytes <= s->budget) break;
+      Pool& p = *pp;
+      if (p.busy == 0 && p.bytes > 0) {
+        p.b.release();
+        s->mem_used -= p.bytes;
+        p.bytes = 0;
+        p.sig = 0;
+        p.free_slots.clear();
+        p.slot_req.clear();
+        s->group_dirty = true;
+      }
+    }
+    if (s->mem_used + bytes > s->budget) { *out = -1; return GP_OK; }  // wait for slots to free up
+  }
+  std::unique_ptr<Pool> p(new Pool());
+  p->sig = m->sig;
+  p->all_rev = m->all_rev;
+  p->dt = r.dt;
+  std::vector<const gp_model*> lanes(cap, m);
+  int st = bucket_build(p->b, lanes);
+  if (st) return st;
+  hipError_t e = hipMalloc((void**)&p->b.d_active, sizeof(int) * Bp);
+  if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(active): %s", hipGetErrorString(e));
+  e = hipMemset(p->b.d_active, 0, sizeof(int) * Bp);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "hipMemset(active): %s", hipGetErrorString(e));
+  p->slot_req.assign(cap, -1);
+  for (int l = cap - 1; l >= 0; --l) p->free_slots.push_back(l);
+  p->bytes = bytes;
+  s->mem_used += bytes;
+  s->pools.push_back(std::move(p));
+  s->group_dirty = true;
+  *out = (int)s->pools.size() - 1;
+  return GP_OK;
+}
+
+int rebuild_group(gp_server* s) {
+  GroupTable t{};
+  const int block = s->o.block_size;
+  int blocks = 0;
+  for (auto& pp : s->pools) {
+    Pool& p = *pp;
+    if (p.bytes == 0) continue;  // released
+    if (t.n >= kMaxGroup) return fail(GP_ERR_UNSUPPORTED, "more than %d live pools", kMaxGroup);
+    BucketArgs a = bucket_args(p.b, 1, nullptr);
+    a.dt = p.dt;
+    t.args[t.n] = a;
+    t.topo[t.n] = p.b.fast_topo;
+    t.block_start[t.n] = blocks;
+    blocks += (p.b.B + block - 1) / block;
+    ++t.n;
+  }
+  t.block_start[t.n] = blocks;
+  if (!s->d_group) {
+    hipError_t e = hipMalloc((void**)&s->d_group, sizeof(GroupTable));
+    if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(group): %s", hipGetErrorString(e));
+  }
+  hipError_t e = hipMemcpy(s->d_group, &t, sizeof t, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "hipMemcpy(group): %s", hipGetErrorString(e));
+  s->group_blocks = blocks;
+  s->group_dirty = false;
+  return GP_OK;
+}
+
+// Pack the admissions of one round and write them into their pools.
+int admit_round(gp_server* s, std::map<int, std::vector<int64_t>>& adm) {
+  // layout per pool: lanes, kind, steps (ints) then bp, cp, hp, q, v, tau (doubles)
+  struct Off { size_t lanes, kind, steps, bp, cp, hp, q, v, tau; int K; };
+  std::map<int, Off> offs;
+  size_t at = 0;
+  for (auto& kv : adm) {
+    const Bucket& b = s->pools[kv.first]->b;
+    const size_t K = kv.second.size();
+    Off o{};
+    o.K = (int)K;
+    o.lanes = at; at += round_up<double>(K * sizeof(int));
+    o.kind = at; at += round_up<double>(K * b.nb * sizeof(int));
+    o.steps = at; at += round_up<double>(K * sizeof(int));
+    o.bp = at; at += K * b.nb * NBF * sizeof(double);
+    o.cp = at; at += K * b.nc * NCF * sizeof(double);
+    o.hp = at; at += K * b.nh * NHF * sizeof(double);
+    o.q = at; at += K * std::max(1, b.nq) * sizeof(double);
+    o.v = at; at += K * std::max(1, b.nv) * sizeof(double);
+    o.tau = at; at += K * std::max(1, b.nv) * sizeof(double);
+    offs[kv.first] = o;
+  }
+  int st = ensure_stage(s, at);
+  if (st) return st;
+  char* h = s->h_stage;
+  for (auto& kv : adm) {
+    const Off& o = offs[kv.first];
+    const Bucket& b = s->pools[kv.first]->b;
+    for (int k = 0; k < o.K; ++k) {
+      const Request& r = s->req[kv.second[k]];
+      const gp_model* m = r.model;
+      ((int*)(h + o.lanes))[k] = r.slot;
+      for (int i = 0; i < b.nb; ++i) ((int*)(h + o.kind))[k * b.nb + i] = m->kind[i];
+      ((int*)(h + o.steps))[k] = (int)r.horizon;
+      std::memcpy(h + o.bp + (size_t)k * b.nb * NBF * 8, m->bp.data(), (size_t)b.nb * NBF * 8);
+      if (b.nc) std::memcpy(h + o.cp + (size_t)k * b.nc * NCF * 8, m->cp.data(), (size_t)b.nc * NCF * 8);
+      if (b.nh) std::memcpy(h + o.hp + (size_t)k * b.nh * NHF * 8, m->hp.data(), (size_t)b.nh * NHF * 8);
+      std::memcpy(h + o.q + (size_t)k * b.nq * 8, r.q0.data(), (size_t)b.nq * 8);
+      std::memcpy(h + o.v + (size_t)k * b.nv * 8, r.v0.data(), (size_t)b.nv * 8);
+      std::memcpy(h + o.tau + (size_t)k * b.nv * 8, r.tau.data(), (size_t)b.nv * 8);
+    }
+  }
+  hipError_t e = hipMemcpyAsync(s->d_stage, s->h_stage, at, hipMemcpyHostToDevice, s->stream);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "admission copy: %s", hipGetErrorString(e));
+  for (auto& kv : adm) {
+    const Off& o = offs[kv.first];
+    char* d = s->d_stage;
+    AdmitBatch x{o.K, (const int*)(d + o.lanes), (const int*)(d + o.kind), (const int*)(d + o.steps),
+                 (const double*)(d + o.bp), (const double*)(d + o.cp), (const double*)(d + o.hp),
+                 (const double*)(d + o.q), (const double*)(d + o.v), (const double*)(d + o.tau)};
+    int err = launch_admit(pool_ptrs(*s->pools[kv.first]), x, s->stream);
+    if (err) return fail(GP_ERR_DEVICE, "admit launch: %s", hipGetErrorString((hipError_t)err));
+  }
+  // the pinned staging is reused next round: wait for the copy
+  e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "admission sync: %s", hipGetErrorString(e));
+  return GP_OK;
+}
+
+int retire_round(gp_server* s, std::map<int, std::vector<int64_t>>& fin) {
+  struct Off { size_t lanes, q, v, fault; int K; };
+  std::map<int, Off> offs;
+  size_t at = 0;
+  for (auto& kv : fin) {
+    const Bucket& b = s->pools[kv.first]->b;
+    const size_t K = kv.second.size();
+    Off o{};
+    o.K = (int)K;
+    o.lanes = at; at += round_up<double>(K * sizeof(int));
+    o.q = at; at += K * std::max(1, b.nq) * sizeof(double);
+    o.v = at; at += K * std::max(1, b.nv) * sizeof(double);
+    o.fault = at; at += round_up<double>(K * sizeof(int));
+    offs[kv.first] = o;
+  }
+  int st = ensure_stage(s, at);
+  if (st) return st;
+  char* h = s->h_stage;
+  for (auto& kv : fin) {
+    const Off& o = offs[kv.first];
+    for (int k = 0; k < o.K; ++k) ((int*)(h + o.lanes))[k] = s->req[kv.second[k]].slot;
+  }
+  hipError_t e = hipMemcpyAsync(s->d_stage, s->h_stage, at, hipMemcpyHostToDevice, s->stream);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "retire copy: %s", hipGetErrorString(e));
+  for (auto& kv : fin) {
+    const Off& o = offs[kv.first];
+    char* d = s->d_stage;
+    GatherBatch g{o.K, (const int*)(d + o.lanes), (double*)(d + o.q), (double*)(d + o.v), (int*)(d + o.fault)};
+    int err = launch_gather(pool_ptrs(*s->pools[kv.first]), g, s->stream);
+    if (err) return fail(GP_ERR_DEVICE, "gather launch: %s", hipGetErrorString((hipError_t)err));
+  }
+  e = hipMemcpyAsync(s->h_stage, s->d_stage, at, hipMemcpyDeviceToHost, s->stream);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "result copy: %s", hipGetErrorString(e));
+  e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "round sync: %s", hipGetErrorString(e));
+  const double t = s->now();
+  for (auto& kv : fin) {
+    const Off& o = offs[kv.first];
+    Pool& p = *s->pools[kv.first];
+    for (int k = 0; k < o.K; ++k) {
+      Request& r = s->req[kv.second[k]];
+      const int nq = p.b.nq, nv = p.b.nv;
+      r.q.assign((const double*)(h + o.q) + (size_t)k * nq, (const double*)(h + o.q) + (size_t)(k + 1) * nq);
+      r.v.assign((const double*)(h + o.v) + (size_t)k * nv, (const double*)(h + o.v) + (size_t)(k + 1) * nv);
+      r.fault = ((const int*)(h + o.fault))[k];
+      r.done_t = t;
+      r.finished = true;
+      p.slot_req[r.slot] = -1;
+      p.free_slots.push_back(r.slot);
+      --p.busy;
+      --s->running;
+      s->completed += 1;
+      s->done.push_back(kv.second[k]);
+    }
+  }
+  return GP_OK;
+}
+
+int run_rounds(gp_server* s, int max_rounds, int* rounds_done) {
+  int rounds = 0;
+  const int C = s->o.chunk_steps;
+  for (;;) {
+    if (max_rounds > 0 && rounds >= max_rounds) break;
+    const double now = s->now();
+    while (!s->future.empty() && s->future.begin()->first <= now) {
+      s->ready.push_back(s->future.begin()->second);
+      s->future.erase(s->future.begin());
+    }
+    // admissions: every ready request that finds a slot, in arrival order
+    std::map<int, std::vector<int64_t>> adm;
+    std::deque<int64_t> still;
+    for (int64_t id : s->ready) {
+      Request& r = s->req[id];
+      int pi = -1;
+      int st = find_pool(s, r, &pi);
+      if (st) return st;
+      if (pi < 0) { still.push_back(id); continue; }
+      Pool& p = *s->pools[pi];
+      r.pool = pi;
+      r.slot = p.free_slots.back();
+      p.free_slots.pop_back();
+      p.slot_req[r.slot] = id;
+      ++p.busy;
+      r.remaining = r.horizon;
+      adm[pi].push_back(id);
+      ++s->running;
+      s->admitted += 1;
+    }
+    s->ready.swap(still);
+    if (!adm.empty()) {
+      int st = admit_round(s, adm);
+      if (st) return st;
+    }
+    if (s->running == 0) {
+      if (s->ready.empty() && s->future.empty()) break;
+      if (!s->ready.empty()) return fail(GP_ERR_BUSY, "queued requests do not fit the memory budget");
+      const double wait = s->future.begin()->first - s->now();
+      if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
+      continue;
+    }
+    if (s->group_dirty) {
+      int st = rebuild_group(s);
+      if (st) return st;
+    }
+    const double ta = s->now();
+    int e = launch_grouped_step(s->d_group, s->group_blocks, s->o.precision, 0.0, C, s->o.block_size, s->stream);
+    if (e) return fail(GP_ERR_DEVICE, "grouped launch: %s", hipGetErrorString((hipError_t)e));
+    // host mirror of the kernel's count-down
+    std::map<int, std::vector<int64_t>> fin;
+    for (size_t pi = 0; pi < s->pools.size(); ++pi) {
+      Pool& p = *s->pools[pi];
+      if (!p.busy) continue;
+      for (size_t l = 0; l < p.slot_req.size(); ++l) {
+        const int64_t id = p.slot_req[l];
+        if (id < 0) continue;
+        Request& r = s->req[id];
+        const int64_t k = std::min<int64_t>(C, r.remaining);
+        r.remaining -= k;
+        s->env_steps += (double)k;
+        if (r.remaining == 0) fin[(int)pi].push_back(id);
+      }
+    }
+    if (!fin.empty()) {
+      int st = retire_round(s, fin);
+      if (st) return st;
+    } else {
+      hipError_t he = hipStreamSynchronize(s->stream);
+      if (he != hipSuccess) return fail(GP_ERR_DEVICE, "round sync: %s", hipGetErrorString(he));
+    }
+    s->busy_s += s->now() - ta;
+    s->rounds += 1;
+    ++rounds;
+  }
+  if (rounds_done) *rounds_done = rounds;
+  return GP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gp_server_create(const gp_server_opts* opts, gp_server** out) {
+  if (!out) return fail(GP_ERR_INVALID, "null argument");
+  std::unique_ptr<gp_server> s(new (std::nothrow) gp_server());
+  if (!s) return fail(GP_ERR_NOMEM, "out of host memory");
+  if (opts) s->o = *opts;
+  if (s->o.precision != GP_PREC_F64 && s->o.precision != GP_PREC_MIXED)
+    return fail(GP_ERR_INVALID, "unknown precision %d", s->o.precision);
+  if (s->o.chunk_steps <= 0) s->o.chunk_steps = 100;
+  if (s->o.slots_per_pool <= 0) s->o.slots_per_pool = 4096;
+  if (s->o.block_size <= 0) s->o.block_size = 128;
+  if (s->o.block_size % 64 || s->o.block_size > 256) return fail(GP_ERR_INVALID, "block_size must be 64, 128, 192 or 256");
+  if (s->o.mem_budget_bytes < 0) return fail(GP_ERR_INVALID, "negative memory budget");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(GP_ERR_DEVICE, "no HIP device");
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return fail(GP_ERR_DEVICE, "hipMemGetInfo failed");
+  s->budget = s->o.mem_budget_bytes > 0 ? std::min<size_t>((size_t)s->o.mem_budget_bytes, free_b) : free_b / 2;
+  if (hipStreamCreate(&s->stream) != hipSuccess) return fail(GP_ERR_DEVICE, "hipStreamCreate failed");
+  s->t0 = std::chrono::steady_clock::now();
+  *out = s.release();
+  return GP_OK;
+}
+
+void gp_server_destroy(gp_server* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  s->pools.clear();
+  if (s->d_group) (void)hipFree(s->d_group);
+  if (s->d_stage) (void)hipFree(s->d_stage);
+  if (s->h_stage) (void)hipHostFree(s->h_stage);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+int gp_server_clock(const gp_server* s, double* now_s) {
+  if (!s || !now_s) return fail(GP_ERR_INVALID, "null argument");
+  *now_s = s->now();
+  return GP_OK;
+}
+
+int gp_server_submit(gp_server* s, const gp_model* m, const double* q0, const double* v0, const double* tau,
+                     int64_t horizon, double dt, double arrival_s, int64_t* id) {
+  if (!s || !m || !q0 || !v0 || !id) return fail(GP_ERR_INVALID, "null argument");
+  if (m->fast_topo < 0) return fail(GP_ERR_UNSUPPORTED, "serving needs a topology with a FAST kernel");
+  if (horizon < 0 || horizon > (int64_t)1 << 30) return fail(GP_ERR_INVALID, "horizon out of range");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(GP_ERR_INVALID, "dt must be positive");
+  try {
+    Request r;
+    r.model = m;
+    r.q0.assign(q0, q0 + m->nq);
+    r.v0.assign(v0, v0 + m->nv);
+    r.tau.assign(m->nv, 0.0);
+    if (tau) r.tau.assign(tau, tau + m->nv);
+    r.horizon = horizon;
+    r.dt = dt;
+    r.arrival = arrival_s < 0 ? s->now() : arrival_s;
+    const int64_t rid = s->next_id++;
+    if (horizon == 0) {  // nothing to run: complete at once
+      r.q = r.q0;
+      r.v = r.v0;
+      r.finished = true;
+      r.done_t = std::max(r.arrival, s->now());
+      s->req.emplace(rid, std::move(r));
+      s->done.push_back(rid);
+      s->completed += 1;
+    } else {
+      s->future.emplace(r.arrival, rid);
+      s->req.emplace(rid, std::move(r));
+    }
+    *id = rid;
+    return GP_OK;
+  } catch (const std::bad_alloc&) {
+    return fail(GP_ERR_NOMEM, "out of host memory");
+  }
+}
+
+int gp_server_run(gp_server* s, int32_t max_rounds, int32_t* rounds_done) {
+  if (!s) return fail(GP_ERR_INVALID, "null server");
+  try {
+    return run_rounds(s, max_rounds, rounds_done);
+  } catch (const std::bad_alloc&) {
+    return fail(GP_ERR_NOMEM, "out of host memory");
+  }
+}
+
+int gp_server_pending(const gp_server* s, int64_t* queued, int64_t* running) {
+  if (!s) return fail(GP_ERR_INVALID, "null server");
+  if (queued) *queued = (int64_t)(s->future.size() + s->ready.size());
+  if (running) *running = s->running;
+  return GP_OK;
+}
+
+int gp_server_completed(gp_server* s, int64_t* ids, int64_t max_ids, int64_t* n) {
+  if (!s || !n) return fail(GP_ERR_INVALID, "null argument");
+  const int64_t k = std::min<int64_t>(max_ids, (int64_t)s->done.size());
+  for (int64_t i = 0; i < k; ++i) ids[i] = s->done[i];
+  s->done.erase(s->done.begin(), s->done.begin() + k);
+  *n = k;
+  return GP_OK;
+}
+
+int gp_server_result(gp_server* s, int64_t id, double* q, double* v, double* latency_s, int32_t* fault) {
+  if (!s) return fail(GP_ERR_INVALID, "null server");
+  auto it = s->req.find(id);
+  if (it == s->req.end()) return fail(GP_ERR_INVALID, "unknown request %lld", (long long)id);
+  Request& r = it->second;
+  if (!r.finished) return fail(GP_ERR_BUSY, "request %lld has not completed", (long long)id);
+  if (q) std::copy(r.q.begin(), r.q.end(), q);
+  if (v) std::copy(r.v.begin(), r.v.end(), v);
+  if (latency_s) *latency_s = r.done_t - r.arrival;
+  if (fault) *fault = r.fault;
+  s->req.erase(it);
+  return GP_OK;
+}
+
+int gp_server_stats(const gp_server* s, double* st) {
+  if (!s || !st) return fail(GP_ERR_INVALID, "null argument");
+  st[0] = s->rounds;
+  st[1] = s->env_steps;
+  st[2] = s->busy_s;
+  st[3] = s->admitted;
+  st[4] = s->completed;
+  st[5] = (double)std::count_if(s->pools.begin(), s->pools.end(), [](const std::unique_ptr<Pool>& p) { return p->bytes > 0; });
+  st[6] = (double)s->mem_used;
+  st[7] = (double)s->budget;
+  return GP_OK;
+}
+
+}  // extern "C"

@@ -214,6 +214,48 @@ int gp_batch_clear_faults(gp_batch* batch);
  * launch): total milliseconds and number of kernel launches. */
 int gp_batch_last_timing(const gp_batch* batch, double* ms, int64_t* launches);
 
+/* ---- serving ----------------------------------------------------------------
+ * No counterpart in the reference, which steps one mechanism in-process
+ * (simulate.rs:87). A request is one rollout: a model, its initial state, held
+ * torques, a horizon in steps and dt. The server keeps every admitted request's
+ * state resident in device memory, in a pool per (topology, dt) whose lanes are
+ * slots; between chunks of chunk_steps steps it admits queued requests into
+ * free slots (continuous batching), creating a pool only while the memory
+ * budget allows, and runs one grouped launch per chunk over every pool. Each
+ * request's result is its final (q, v) and its latency = completion - arrival
+ * on the server clock. Topologies without a FAST kernel are rejected
+ * (GP_ERR_UNSUPPORTED). A model must outlive its requests. */
+typedef struct gp_server_opts {
+  int32_t precision;        /* GP_PREC_*                                      */
+  int32_t chunk_steps;      /* steps per round (0 = 100)                      */
+  int32_t slots_per_pool;   /* lanes of one pool (0 = 4096)                   */
+  int32_t block_size;       /* threads per workgroup (0 = 128)                */
+  int64_t mem_budget_bytes; /* pools' device memory (0 = half the free memory) */
+  int32_t _reserved[4];
+} gp_server_opts;
+typedef struct gp_server gp_server;
+
+int gp_server_create(const gp_server_opts* opts, gp_server** out);
+void gp_server_destroy(gp_server* server);
+/* seconds since the server was created */
+int gp_server_clock(const gp_server* server, double* now_s);
+/* Queue a request that arrives at arrival_s on the server clock (< 0: now). */
+int gp_server_submit(gp_server* server, const gp_model* model, const double* q0, const double* v0,
+                     const double* tau, int64_t horizon_steps, double dt, double arrival_s,
+                     int64_t* request_id);
+/* Run rounds until nothing is queued or running (or max_rounds > 0 rounds). */
+int gp_server_run(gp_server* server, int32_t max_rounds, int32_t* rounds_done);
+int gp_server_pending(const gp_server* server, int64_t* queued, int64_t* running);
+/* Ids of requests completed since the last call (up to max_ids). */
+int gp_server_completed(gp_server* server, int64_t* ids, int64_t max_ids, int64_t* n_ids);
+/* Final state of a completed request (then forgotten); fault = 1 if the
+ * instance produced a non-finite value. */
+int gp_server_result(gp_server* server, int64_t request_id, double* q, double* v, double* latency_s,
+                     int32_t* fault);
+/* [rounds, env_steps, busy_seconds, admitted, completed, pools, pool_bytes,
+ *  budget_bytes] */
+int gp_server_stats(const gp_server* server, double* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,171 @@
+"""GPU rollout tests: energy drift, active contact sets, fault flags, RK2/RK4,
+the simulate loop, hipGraph replays and grouped launches."""
+import os
+
+import numpy as np
+import pytest
+
+from gpu_helpers import rel_err, step_errors
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _descs(w):
+    out = []
+    for d, c in zip(w["models"], w["counts"]):
+        out.extend([d] * int(c))
+    return out
+
+
+def _energies(oracle, d, q, v, nq, nv):
+    n = len(q) // nq
+    out = np.empty(n)
+    for i in range(n):
+        ke, pe = oracle.energy(d, q[nq * i:nq * (i + 1)], v[nv * i:nv * (i + 1)])
+        out[i] = ke + pe
+    return out
+
+
+@pytest.mark.parametrize("precision,tol_short,tol_stat", [(0, 1e-8, 0.05), (1, 1e-4, 0.10)])
+def test_acrobot_energy_drift_matches_oracle(gp, oracle, precision, tol_short, tol_stat):
+    """Conservative acrobots (zero torque), semi-implicit Euler, dt 1e-3,
+    2000 steps, energy = KE + sum m g z_com sampled every 100 steps. The
+    acrobots start at q ~ U(-pi, pi) and are chaotic, so trajectories are
+    compared per instance over the first 500 steps (drift within tol_short of
+    the energy scale max(|E0|, KE0, 1)) and over all 2000 steps through the
+    batch's mean and max drift (within tol_stat relative)."""
+    from gorilla_physics_amd import batch, workloads
+    w = workloads.acrobot(n=128)
+    d = w["models"][0]
+    print(f"case energy drift precision={precision}", flush=True)
+    b, _ = batch.batch_from_workload(w, precision=precision, kernel=2)
+    qg, vg = w["q"].copy(), w["v"].copy()
+    qo, vo = w["q"].copy(), w["v"].copy()
+    e0 = _energies(oracle, d, qo, vo, 2, 2)
+    ke0 = np.array([oracle.energy(d, qo[2 * i:2 * i + 2], vo[2 * i:2 * i + 2])[0] for i in range(128)])
+    scale = np.maximum.reduce([np.abs(e0), ke0, np.ones(128)])
+    dg = np.zeros(128)
+    do = np.zeros(128)
+    for k in range(20):
+        b.step(1e-3, 0, 100)
+        qg, vg = b.get_state()
+        qo, vo, _f = oracle.rollout([d] * 128, qo, vo, None, 1e-3, 100, threads=THREADS)
+        dg = np.maximum(dg, np.abs(_energies(oracle, d, qg, vg, 2, 2) - e0))
+        do = np.maximum(do, np.abs(_energies(oracle, d, qo, vo, 2, 2) - e0))
+        if k == 4:
+            short = np.max(np.abs(dg - do) / scale)
+    rg, ro = dg / scale, do / scale
+    print(f"500 steps max |diff| {short:.3g}; 2000 steps drift mean gpu {rg.mean():.4g} oracle {ro.mean():.4g}, "
+          f"max gpu {rg.max():.4g} oracle {ro.max():.4g}", flush=True)
+    assert short < tol_short
+    assert abs(rg.mean() - ro.mean()) <= tol_stat * ro.mean()
+    assert abs(rg.max() - ro.max()) <= tol_stat * ro.max() + 1e-6
+    assert ro.max() < 0.1  # the reference's own energy tolerance (energy.rs:125)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cube_active_contact_sets(gp, oracle, precision):
+    """Cubes dropped 1-5 cm onto the plane with random orientation and spin:
+    after every 100 steps (dt 1e-4) of a rollout from the same state on the
+    GPU and on the oracle, the sets of corners with d <= 1e-8 agree for every
+    corner outside the band |d - 1e-8| <= band."""
+    from gorilla_physics_amd import batch, workloads
+    w = workloads.cubes(n=256, seed=3)
+    q = w["q"].copy()
+    d = w["models"][0]
+    target = np.random.default_rng(0).uniform(0.01, 0.05, 256)  # lowest corner 1-5 cm above the plane
+    lows = np.array([oracle.contact_distances(d, q[7 * i:7 * i + 7]).min() for i in range(256)])
+    q[6::7] += target - lows
+    w["q"] = q
+    print(f"case active sets precision={precision}", flush=True)
+    b, _ = batch.batch_from_workload(w, precision=precision, kernel=2)
+    qo, vo = q.copy(), w["v"].copy()
+    band = 1e-9 if precision == 0 else 1e-6
+    mismatches = 0
+    touched = 0
+    for _ in range(30):
+        b.step(1e-4, 0, 100)
+        qg, _vg = b.get_state()
+        qo, vo, _f = oracle.rollout([d] * 256, qo, vo, None, 1e-4, 100, threads=THREADS)
+        for i in range(256):
+            dg = oracle.contact_distances(d, qg[7 * i:7 * i + 7])
+            dr = oracle.contact_distances(d, qo[7 * i:7 * i + 7])
+            outside = np.abs(dr - 1e-8) > band
+            mismatches += int(np.sum((dg <= 1e-8)[outside] != (dr <= 1e-8)[outside]))
+            touched += int(np.sum(dr <= 1e-8))
+    print(f"active corner-samples {touched}, mismatches outside the {band:g} m band: {mismatches}", flush=True)
+    assert touched > 1000 and mismatches == 0
+
+
+def test_fault_flags_match_oracle_divergence(gp, oracle):
+    """The FP64 kernel flags exactly the arms on which the oracle's own
+    rollout becomes non-finite (constant torques on light links)."""
+    from gorilla_physics_amd import batch, workloads
+    w = workloads.arms(n=4096)
+    print("case fault flags", flush=True)
+    b, _ = batch.batch_from_workload(w, precision=0, kernel=2)
+    b.step(1e-3, 0, 400)
+    flags = b.faults().astype(bool)
+    qo, vo, _f = oracle.rollout(w["models"], w["q"], w["v"], w["tau"], 1e-3, 400, threads=THREADS)
+    bad = ~(np.isfinite(qo.reshape(-1, 7)).all(1) & np.isfinite(vo.reshape(-1, 7)).all(1))
+    print(f"oracle diverged {bad.sum()}, gpu flagged {flags.sum()}, both {(bad & flags).sum()}", flush=True)
+    assert bad.sum() > 0
+    assert np.array_equal(bad, flags)
+
+
+@pytest.mark.parametrize("integrator", [1, 2])
+@pytest.mark.parametrize("name", ["mixed", "cubes"])
+def test_runge_kutta_step_parity(gp, oracle, integrator, name):
+    """RK2 / RK4 (integrators.rs:177-225) run on the plain kernel: one step
+    against the oracle's, FP64 bound."""
+    from gorilla_physics_amd import batch, workloads
+    w = getattr(workloads, name)(n=256)
+    if name == "cubes":
+        w["q"] = w["q"].copy()
+        w["q"][6::7] = 0.46  # corners in contact
+    print(f"case rk{2 * integrator} {name}", flush=True)
+    b, _ = batch.batch_from_workload(w, precision=0)
+    q0, v0 = b.get_state()
+    b.step(w["dt"], integrator, 1)
+    q1, v1 = b.get_state()
+    a1 = b.get_vdot()
+    eq, ev, ea = step_errors(oracle, _descs(w), q0, v0, w["tau"], q1, v1, a1, w["dt"], integrator)
+    print(f"max e(q') {eq.max():.3g} e(v') {ev.max():.3g} e(a1) {ea.max():.3g}", flush=True)
+    assert eq.max() < 1e-10 and ev.max() < 1e-10 and ea.max() < 1e-10
+
+
+def test_simulate_loop_matches_oracle(gp, oracle):
+    """simulate() runs the reference's float-accumulated loop: 0.2 s at 1e-3
+    records 202 states (201 steps + initial) equal to the oracle's."""
+    from gorilla_physics_amd import builders
+    m, l = 1.0, 1.0
+    st = builders.build_double_pendulum(m, np.diag([0, m * l * l / 3, m * l * l / 3]), [m * l / 2, 0, 0],
+                                        gp.Isometry3.identity(), gp.Isometry3.translation_xyz(l, 0, 0), [0, 1, 0])
+    st.update([0.3, -0.2], [0.1, 0.4])
+    qs, vs = gp.simulate(st, 0.2, 1e-3, lambda s: [0.5, -0.25], gp.Integrator.SemiImplicitEuler)
+    rq, rv = oracle.simulate(st.model_desc(), 0.2, 1e-3, [0.3, -0.2], [0.1, 0.4], [0.5, -0.25])
+    assert len(qs) == len(rq) == 202
+    assert np.allclose(np.array(qs), rq, rtol=0, atol=1e-11) and np.allclose(np.array(vs), rv, rtol=0, atol=1e-9)
+
+
+def test_graph_replays_and_grouped_launch_are_exact(gp):
+    """A hipGraph-captured rollout replayed several times (with eager work in
+    between) and a grouped launch over all buckets give bit-identical states
+    to plain per-bucket launches."""
+    from gorilla_physics_amd import batch, workloads
+    w = workloads.mixed(n=2048)
+    print("case graph/grouped", flush=True)
+    ref, _ = batch.batch_from_workload(w, precision=1, separate_launches=1, use_graph=0)
+    g, _ = batch.batch_from_workload(w, precision=1, use_graph=1)
+    assert g.n_buckets > 2
+    for rep in range(3):
+        ref.step(1e-3, 0, 250)
+        g.step(1e-3, 0, 250)
+        g.dynamics()  # eager work on the same batch between graph replays
+        qa, va = ref.get_state()
+        qb, vb = g.get_state()
+        assert np.array_equal(qa, qb) and np.array_equal(va, vb), rep
+    ms, launches = g.last_timing()
+    assert launches == 3 and ms > 0
