@@ -57,12 +57,20 @@ def make_workload(name, n):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_variant(w, models, steps, warmup, **opts):
-    """Returns (env-steps/s, ms per step, lib-timed ms, launches)."""
+def run_variant(w, models, steps, warmup, autotune=False, **opts):
+    """Returns env-steps/s, ms per step, lib-timed ms, launches, faults and the
+    options the batch ran with (after gp_batch_autotune when asked; the tuning
+    runs before the warmup, outside the timed region, and restores the state)."""
     from gorilla_physics_amd import batch
     b = batch.Batch(models, w["counts"], **opts)
     b.set_state(w["q"], w["v"])
     b.set_tau(w["tau"])
+    tuned_ms = None
+    if autotune:
+        _, tuned_ms = b.autotune(w["dt"], 0, trial_steps=200)
+    o = b.get_opts()
+    chosen = dict(block_size=o.block_size, lanes_per_mech=o.lanes_per_mech, lanes_per_tree=o.lanes_per_tree,
+                  steps_per_launch=o.steps_per_launch, use_graph=o.use_graph)
     if warmup:
         b.step(w["dt"], 0, warmup)
     b.sync()
@@ -78,7 +86,7 @@ def run_variant(w, models, steps, warmup, **opts):
     sec = t1 - t0
     del b
     return dict(value=n * steps / sec, ms_per_step=1e3 * sec / steps, lib_ms=lib_ms, launches=launches,
-                faults=faults, instances=n)
+                faults=faults, instances=n, opts=chosen, autotune_trial_ms=tuned_ms)
 
 
 def serving_requests(n, seed=0):
@@ -200,6 +208,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--autotune", type=int, default=1, help="gp_batch_autotune before the warmup")
+    ap.add_argument("--lanes-tree", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -237,7 +247,8 @@ def main():
     models = [batch.Model(d) for d in w["models"]]
     log(f"workload {args.workload}: {sum(w['counts'])} instances")
     spl = args.steps_per_launch or min(args.steps, 100)
-    common = dict(block_size=args.block, lanes_per_mech=args.lanes, steps_per_launch=spl, use_graph=args.graph)
+    common = dict(block_size=args.block, lanes_per_mech=args.lanes, lanes_per_tree=args.lanes_tree,
+                  steps_per_launch=spl, use_graph=args.graph, autotune=bool(args.autotune))
     mixed = run_variant(w, models, args.steps, args.warmup, precision=abi.GP_PREC_MIXED, kernel=abi.GP_KERNEL_AUTO,
                         **common)
     log("mixed", mixed)
@@ -248,7 +259,7 @@ def main():
         log("fast f64", f64)
         plain_steps = max(10, args.steps // 10)
         plain = run_variant(w, models, plain_steps, min(args.warmup, 10), precision=abi.GP_PREC_F64,
-                            kernel=abi.GP_KERNEL_PLAIN, use_graph=args.graph)
+                            kernel=abi.GP_KERNEL_PLAIN, use_graph=args.graph)  # one step per launch: nothing to tune
         log("plain f64", plain)
         extra = dict(fast_f64_env_steps_per_s=f64["value"], plain_f64_env_steps_per_s=plain["value"],
                      speedup_mixed_vs_plain_f64=mixed["value"] / plain["value"],
@@ -259,6 +270,7 @@ def main():
     roof = None
     if flops_per_env_step:
         launch_s = mixed["lib_ms"] / 1e3 / max(1, mixed["launches"])
+        spl = mixed["opts"]["steps_per_launch"] or spl
         flops = flops_per_env_step * mixed["instances"] * spl
         achieved = flops / launch_s / 1e12
         traffic = work.get(args.workload, {}).get("hbm_bytes_per_launch")
@@ -287,9 +299,10 @@ def main():
             "mixed": "32,768 chains 1-12 DOF + floating quadrupeds, dt=1e-3, semi-implicit Euler, mixed precision",
             "cubes": "16,384 free cubes x 8 corner contacts on z=0, dt=1e-4, semi-implicit Euler, mixed precision",
             "acrobot": "1,024 acrobots, dt=1e-3, semi-implicit Euler, mixed precision"}[args.workload],
-            "instances": mixed["instances"], "steps_per_launch": spl, "hip_graph": bool(args.graph),
-            "error_bound": "teacher-forced per-step e(q'),e(v') <= 1e-6, e(qdd) <= 5e-3 vs FP64 oracle "
-                           "(FP64 kernels: <= 1e-10), tests/test_gpu_parity.py"},
+            "instances": mixed["instances"], "steps_per_launch": mixed["opts"]["steps_per_launch"] or spl,
+            "hip_graph": bool(args.graph), "launch_opts": mixed["opts"], "autotuned": bool(args.autotune),
+            "error_bound": "teacher-forced per-step vs FP64 oracle: e(q') <= 1e-5, e(v'), e(qdd) <= 2e-3, "
+                           "p99 e(qdd) <= 1e-4 (FP64 kernels: all <= 1e-10), tests/test_gpu_parity.py"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "faults": mixed["faults"],

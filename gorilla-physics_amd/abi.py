@@ -86,7 +86,7 @@ class gp_batch_opts(C.Structure):
         ("lanes_per_mech", C.c_int32),
         ("use_graph", C.c_int32),
         ("separate_launches", C.c_int32),
-        ("_reserved", C.c_int32),
+        ("lanes_per_tree", C.c_int32),
     ]
 
 
@@ -97,7 +97,9 @@ class gp_server_opts(C.Structure):
         ("slots_per_pool", C.c_int32),
         ("block_size", C.c_int32),
         ("mem_budget_bytes", C.c_int64),
-        ("_reserved", C.c_int32 * 4),
+        ("lanes_per_mech", C.c_int32),
+        ("lanes_per_tree", C.c_int32),
+        ("_reserved", C.c_int32 * 2),
     ]
 
 
@@ -134,6 +136,7 @@ SIGNATURES = {
     "gp_batch_faults": (C.c_int, [_P, C.POINTER(C.c_uint8)]),
     "gp_batch_clear_faults": (C.c_int, [_P]),
     "gp_batch_last_timing": (C.c_int, [_P, _D, C.POINTER(C.c_int64)]),
+    "gp_batch_autotune": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, C.POINTER(gp_batch_opts), _D]),
     "gp_server_create": (C.c_int, [C.POINTER(gp_server_opts), C.POINTER(_P)]),
     "gp_server_destroy": (None, [_P]),
     "gp_server_clock": (C.c_int, [_P, _D]),

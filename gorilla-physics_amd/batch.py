@@ -36,11 +36,12 @@ class Model:
 
 
 def make_opts(precision=abi.GP_PREC_F64, kernel=abi.GP_KERNEL_AUTO, steps_per_launch=0, block_size=0,
-              lanes_per_mech=0, use_graph=0, separate_launches=0):
+              lanes_per_mech=0, use_graph=0, separate_launches=0, lanes_per_tree=0):
     o = abi.gp_batch_opts()
     o.precision, o.kernel, o.steps_per_launch = precision, kernel, steps_per_launch
     o.block_size, o.lanes_per_mech, o.use_graph = block_size, lanes_per_mech, use_graph
     o.separate_launches = separate_launches
+    o.lanes_per_tree = lanes_per_tree
     return o
 
 
@@ -162,6 +163,15 @@ class Batch:
 
     def clear_faults(self):
         abi.check(self.lib.gp_batch_clear_faults(self.handle), self.lib)
+
+    def autotune(self, dt, integrator=abi.GP_SEMI_IMPLICIT_EULER, trial_steps=0):
+        """gp_batch_autotune: time trial rollouts over lanes / workgroup size /
+        steps per launch, apply the fastest options (state left unchanged).
+        Returns (chosen gp_batch_opts, trial milliseconds)."""
+        o, ms = abi.gp_batch_opts(), C.c_double()
+        abi.check(self.lib.gp_batch_autotune(self.handle, float(dt), int(integrator), int(trial_steps),
+                                             C.byref(o), C.byref(ms)), self.lib)
+        return o, ms.value
 
     def last_timing(self):
         ms, n = C.c_double(), C.c_int64()

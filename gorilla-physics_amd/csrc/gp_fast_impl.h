@@ -366,7 +366,218 @@ __global__ void __launch_bounds__(256) fast_sie_kernel(BucketArgs a, double dt, 
   fast_sie_body<Topo, R, L, CONTACTS, ALLREV>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps);
 }
 
-enum { TOPO_FREE = 100, TOPO_QUAD = 101 };
+// ---------------------------------------------------------------- trees, one lane per leg
+// A floating base with LEGS legs, each a chain of DEPTH one-DOF joints (body
+// 1 + l * DEPTH + k is joint k of leg l; TopoQuad is LEGS = 4, DEPTH = 3).
+// LEGS consecutive lanes hold one instance: each runs the base redundantly and
+// its own leg; the legs' contributions to the base's articulated inertia and
+// bias force are summed across the lanes with xor butterflies (every lane gets
+// the same bits), and every lane solves the base's 6x6 system in FP64.
+template <int LEGS, int DEPTH>
+struct TopoTree {
+  static constexpr int N = 1 + LEGS * DEPTH;
+};
+
+template <class R, int LEGS>
+__device__ __forceinline__ R legs_sum(R x) {
+#pragma unroll
+  for (int off = 1; off < LEGS; off <<= 1) x += __shfl_xor(x, off, LEGS);
+  return x;
+}
+template <class R, int LEGS>
+__device__ __forceinline__ void legs_sum(ai6<R>& I, sv<R>& f) {
+  I.A.xx = legs_sum<R, LEGS>(I.A.xx); I.A.yy = legs_sum<R, LEGS>(I.A.yy); I.A.zz = legs_sum<R, LEGS>(I.A.zz);
+  I.A.xy = legs_sum<R, LEGS>(I.A.xy); I.A.xz = legs_sum<R, LEGS>(I.A.xz); I.A.yz = legs_sum<R, LEGS>(I.A.yz);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) I.B.a[k] = legs_sum<R, LEGS>(I.B.a[k]);
+  I.C.xx = legs_sum<R, LEGS>(I.C.xx); I.C.yy = legs_sum<R, LEGS>(I.C.yy); I.C.zz = legs_sum<R, LEGS>(I.C.zz);
+  I.C.xy = legs_sum<R, LEGS>(I.C.xy); I.C.xz = legs_sum<R, LEGS>(I.C.xz); I.C.yz = legs_sum<R, LEGS>(I.C.yz);
+  f.w.x = legs_sum<R, LEGS>(f.w.x); f.w.y = legs_sum<R, LEGS>(f.w.y); f.w.z = legs_sum<R, LEGS>(f.w.z);
+  f.v.x = legs_sum<R, LEGS>(f.v.x); f.v.y = legs_sum<R, LEGS>(f.v.y); f.v.z = legs_sum<R, LEGS>(f.v.z);
+}
+
+template <class R, int LEGS, int DEPTH, bool ALLREV>
+__device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, double dt, int n_steps) {
+  constexpr int N = TopoTree<LEGS, DEPTH>::N;
+  const int inst = tid / LEGS;
+  const int leg = tid % LEGS;
+  if (inst >= a.B) return;
+  const int Bp = a.Bp;
+  int my_steps = n_steps;
+  if (a.active) {
+    const int left = a.active[inst];
+    if (left <= 0) return;
+    my_steps = min(my_steps, left);
+    if (leg == 0) a.active[inst] = left - my_steps;
+  }
+  const FastParams<TopoTree<LEGS, DEPTH>, R> P{pick<R>(a.bp, a.bpf), Bp, inst};
+  // state: base (q 7, v 6) in every lane, this leg's joints
+  double qb[7], vb[6], tb[6], ab[6];
+  double ql[DEPTH], vl[DEPTH], tl[DEPTH], al[DEPTH];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) qb[k] = a.q[(size_t)k * Bp + inst];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    vb[k] = a.v[(size_t)k * Bp + inst];
+    tb[k] = a.tau[(size_t)k * Bp + inst];
+    ab[k] = a.vdot[(size_t)k * Bp + inst];
+  }
+#pragma unroll
+  for (int k = 0; k < DEPTH; ++k) {
+    const int j = leg * DEPTH + k;
+    ql[k] = a.q[(size_t)(7 + j) * Bp + inst];
+    vl[k] = a.v[(size_t)(6 + j) * Bp + inst];
+    tl[k] = a.tau[(size_t)(6 + j) * Bp + inst];
+    al[k] = a.vdot[(size_t)(6 + j) * Bp + inst];
+  }
+  bool ok = true;
+  for (int s = 0; s < my_steps; ++s) {
+    // ---- pass 1: base, then this leg
+    m3<double> Rq = quat_mat(qb[0], qb[1], qb[2], qb[3]);
+    m3<double> Rtd;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Rtd.a[k] = a.bp[(size_t)((BF_RT + k) * N) * Bp + inst];
+    v3<double> ptd{a.bp[(size_t)(BF_PT * N) * Bp + inst], a.bp[(size_t)((BF_PT + 1) * N) * Bp + inst],
+                   a.bp[(size_t)((BF_PT + 2) * N) * Bp + inst]};
+    const m3<R> Rm = cvt<R>(mul(Rtd, Rq));
+    const m3<R> E0{{Rm.a[0], Rm.a[3], Rm.a[6], Rm.a[1], Rm.a[4], Rm.a[7], Rm.a[2], Rm.a[5], Rm.a[8]}};
+    const v3<R> r0 = cvt<R>(ptd + mul(Rtd, v3<double>{qb[4], qb[5], qb[6]}));
+    const sv<R> V0{{R(vb[0]), R(vb[1]), R(vb[2])}, {R(vb[3]), R(vb[4]), R(vb[5])}};
+    const sv<R> cb0 = crm(V0, V0);
+    const ai6<R> I0 = P.inertia(0);
+    sv<R> pA0 = crf(V0, mul(I0, V0));
+    Body1<R> b[DEPTH];
+    sv<R> vel = V0;
+#pragma unroll
+    for (int k = 0; k < DEPTH; ++k) {
+      const int i = 1 + leg * DEPTH + k;
+      const bool rev = ALLREV || a.kind[i * Bp + inst] == K_REV;
+      b[k].rev = rev;
+      sincos_q(rev ? ql[k] : 0.0, &b[k].s, &b[k].c);
+      b[k].d = rev ? R(0) : R(ql[k]);
+      m3<R> E;
+      v3<R> r;
+      joint_frame(P.Rt(i), P.pt(i), b[k].c, b[k].s, b[k].d, E, r);
+      const R qd = R(vl[k]);
+      sv<R> vj{{R(0), R(0), rev ? qd : R(0)}, {R(0), R(0), rev ? R(0) : qd}};
+      vel = xmotion(E, r, vel) + vj;
+      b[k].pA = crf(vel, mul(P.inertia(i), vel));
+      b[k].cw0 = rev ? qd * vel.w.y : R(0);
+      b[k].cw1 = rev ? -qd * vel.w.x : R(0);
+      b[k].cv0 = qd * (rev ? vel.v.y : vel.w.y);
+      b[k].cv1 = -qd * (rev ? vel.v.x : vel.w.x);
+    }
+    // ---- pass 2: this leg, leaves to its first joint, then the base
+    ai6<R> acc;
+    ai6<R> cIA;
+    sv<R> cpA;
+#pragma unroll
+    for (int k = DEPTH - 1; k >= 0; --k) {
+      const int i = 1 + leg * DEPTH + k;
+      Body1<R>& B = b[k];
+      ai6<R> IA = P.inertia(i);
+      if (k < DEPTH - 1) add_to(IA, acc);
+      B.U = B.rev ? col(IA, 2) : col(IA, 5);
+      B.D = double(B.rev ? IA.A.zz : IA.C.zz);
+      double tq = tl[k];
+      if (!B.rev) {
+        const R sk = P(BF_SK, i);
+        if (sk != R(0)) tq += -double(sk) * (ql[k] - double(P(BF_SL, i)));
+      }
+      B.u = tq - double(B.rev ? B.pA.w.z : B.pA.v.z);
+      ok = ok && (B.D > 0.0);
+      const R inv = R(1.0 / B.D);
+      const sv<R> uD{inv * B.U.w, inv * B.U.v};
+      const ai6<R> Ia = rank1_down(IA, B.U, uD);
+      const R uR = R(B.u);
+      const sv<R> cbv{{B.cw0, B.cw1, R(0)}, {B.cv0, B.cv1, R(0)}};
+      const sv<R> pa = B.pA + mul(Ia, cbv) + sv<R>{uR * uD.w, uR * uD.v};
+      m3<R> E;
+      v3<R> r;
+      joint_frame(P.Rt(i), P.pt(i), B.c, B.s, B.d, E, r);
+      if (k > 0) {
+        acc = xtix(E, r, Ia);
+        b[k - 1].pA = b[k - 1].pA + xtforce(E, r, pa);
+      } else {
+        cIA = xtix(E, r, Ia);
+        cpA = xtforce(E, r, pa);
+      }
+    }
+    legs_sum<R, LEGS>(cIA, cpA);
+    ai6<R> IA0 = I0;
+    add_to(IA0, cIA);
+    pA0 = pA0 + cpA;
+    double D6[6][6], Lf[21], u6[6];
+    to_dense(IA0, D6);
+    ok = chol6(D6, Lf) && ok;
+    u6[0] = tb[0] - double(pA0.w.x); u6[1] = tb[1] - double(pA0.w.y); u6[2] = tb[2] - double(pA0.w.z);
+    u6[3] = tb[3] - double(pA0.v.x); u6[4] = tb[4] - double(pA0.v.y); u6[5] = tb[5] - double(pA0.v.z);
+    // ---- pass 3: base, then this leg
+    const sv<R> ag{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}};
+    sv<R> ap0 = xmotion(E0, r0, ag) + cb0;
+    const double apd[6] = {double(ap0.w.x), double(ap0.w.y), double(ap0.w.z),
+                           double(ap0.v.x), double(ap0.v.y), double(ap0.v.z)};
+    double x[6];
+#pragma unroll
+    for (int rr = 0; rr < 6; ++rr) {
+      double t = u6[rr];
+#pragma unroll
+      for (int cc = 0; cc < 6; ++cc) t -= D6[rr][cc] * apd[cc];
+      x[rr] = t;
+    }
+    chol6_solve(Lf, x);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) ab[t] = x[t];
+    sv<R> accel = ap0 + sv<R>{{R(x[0]), R(x[1]), R(x[2])}, {R(x[3]), R(x[4]), R(x[5])}};
+#pragma unroll
+    for (int k = 0; k < DEPTH; ++k) {
+      const int i = 1 + leg * DEPTH + k;
+      const Body1<R>& B = b[k];
+      m3<R> E;
+      v3<R> r;
+      joint_frame(P.Rt(i), P.pt(i), B.c, B.s, B.d, E, r);
+      sv<R> ap = xmotion(E, r, accel);
+      ap.w.x += B.cw0; ap.w.y += B.cw1; ap.v.x += B.cv0; ap.v.y += B.cv1;
+      const double qdd = (B.u - double(dot6(B.U, ap))) / B.D;
+      al[k] = qdd;
+      if (B.rev) ap.w.z += R(qdd); else ap.v.z += R(qdd);
+      accel = ap;
+    }
+    // ---- semi-implicit Euler (integrators.rs:276-319)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) vb[k] = vb[k] + ab[k] * dt;
+    advance_floating(qb, vb, dt, qb);
+#pragma unroll
+    for (int k = 0; k < DEPTH; ++k) {
+      vl[k] = vl[k] + al[k] * dt;
+      ql[k] += vl[k] * dt;
+    }
+  }
+  if (leg == 0) {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) { a.q[(size_t)k * Bp + inst] = qb[k]; ok = ok && isfinite(qb[k]); }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      a.v[(size_t)k * Bp + inst] = vb[k];
+      a.vdot[(size_t)k * Bp + inst] = ab[k];
+      ok = ok && isfinite(vb[k]) && isfinite(ab[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < DEPTH; ++k) {
+    const int j = leg * DEPTH + k;
+    a.q[(size_t)(7 + j) * Bp + inst] = ql[k];
+    a.v[(size_t)(6 + j) * Bp + inst] = vl[k];
+    a.vdot[(size_t)(6 + j) * Bp + inst] = al[k];
+    ok = ok && isfinite(ql[k]) && isfinite(vl[k]) && isfinite(al[k]);
+  }
+  if (!ok) a.fault[inst] = 1;
+}
+
+template <class R, int LEGS, int DEPTH, bool ALLREV>
+__global__ void __launch_bounds__(256) tree_sie_kernel(BucketArgs a, double dt, int n_steps) {
+  tree_sie_body<R, LEGS, DEPTH, ALLREV>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps);
+}
 
 // ---------------------------------------------------------------- grouped launch
 // Every bucket of a batch in one launch: bucket b owns workgroups
@@ -386,8 +597,13 @@ __device__ __forceinline__ void grouped_dispatch(const BucketArgs& a, int topo, 
     GP_CHAIN_CASE(7) GP_CHAIN_CASE(8) GP_CHAIN_CASE(9) GP_CHAIN_CASE(10) GP_CHAIN_CASE(11) GP_CHAIN_CASE(12)
 #undef GP_CHAIN_CASE
     case TOPO_QUAD:
-      if (ar) fast_sie_body<TopoQuad, R, 1, false, true>(a, tid, dt, n);
-      else fast_sie_body<TopoQuad, R, 1, false, false>(a, tid, dt, n);
+      if (a.lanes == 4) {
+        if (ar) tree_sie_body<R, 4, 3, true>(a, tid, dt, n);
+        else tree_sie_body<R, 4, 3, false>(a, tid, dt, n);
+      } else {
+        if (ar) fast_sie_body<TopoQuad, R, 1, false, true>(a, tid, dt, n);
+        else fast_sie_body<TopoQuad, R, 1, false, false>(a, tid, dt, n);
+      }
       break;
     case TOPO_FREE:
       switch (a.lanes) {
@@ -425,6 +641,15 @@ static int launch_chain(const BucketArgs& a, double dt, int n, int block, hipStr
   return launch_t<Topo, R, 1, false, false>(a, dt, n, block, s);
 }
 
+template <class R, int LEGS, int DEPTH>
+static int launch_tree(const BucketArgs& a, double dt, int n_steps, int block, hipStream_t s) {
+  long threads = (long)a.B * LEGS;
+  dim3 blk(block), grid((unsigned)((threads + block - 1) / block));
+  if (a.all_rev) hipLaunchKernelGGL((tree_sie_kernel<R, LEGS, DEPTH, true>), grid, blk, 0, s, a, dt, n_steps);
+  else hipLaunchKernelGGL((tree_sie_kernel<R, LEGS, DEPTH, false>), grid, blk, 0, s, a, dt, n_steps);
+  return (int)hipGetLastError();
+}
+
 template <class R>
 static int launch_prec(int topo, const BucketArgs& a, double dt, int n, int block, hipStream_t s) {
   switch (topo) {
@@ -440,7 +665,9 @@ static int launch_prec(int topo, const BucketArgs& a, double dt, int n, int bloc
     case 10: return launch_chain<TopoChain<10>, R>(a, dt, n, block, s);
     case 11: return launch_chain<TopoChain<11>, R>(a, dt, n, block, s);
     case 12: return launch_chain<TopoChain<12>, R>(a, dt, n, block, s);
-    case TOPO_QUAD: return launch_chain<TopoQuad, R>(a, dt, n, block, s);
+    case TOPO_QUAD:
+      if (a.lanes == 4) return launch_tree<R, 4, 3>(a, dt, n, block, s);
+      return launch_chain<TopoQuad, R>(a, dt, n, block, s);
     case TOPO_FREE:
       switch (a.lanes) {
         case 2: return launch_t<TopoFree, R, 2, true, false>(a, dt, n, block, s);

@@ -98,6 +98,18 @@ struct GroupTable {
   BucketArgs args[kMaxGroup];
 };
 
+// Compile-time topologies besides the chains (whose id is their body count).
+enum { TOPO_FREE = 100, TOPO_QUAD = 101 };
+
+// Lanes per instance of a FAST bucket: free bodies with contacts split their
+// contact points over lanes_per_mech lanes (0 = 4); the quadruped tree runs one
+// lane per leg unless lanes_per_tree is 1.
+inline int fast_lanes(int fast_topo, int lanes_per_mech, int lanes_per_tree) {
+  if (fast_topo == TOPO_FREE) return lanes_per_mech > 0 ? lanes_per_mech : 4;
+  if (fast_topo == TOPO_QUAD) return lanes_per_tree == 1 ? 1 : 4;
+  return 1;
+}
+
 // Host-side launchers (gp_plain.hip, gp_fast.hip). Each returns a hipError_t as int.
 int launch_grouped_step(const GroupTable* d_table, int n_blocks, int precision, double dt, int n_steps, int block,
                         void* stream);

@@ -340,7 +340,7 @@ static bool bucket_is_fast(const gp_batch_opts& o, const Bucket& b, int integrat
   return o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0 && integrator == GP_SEMI_IMPLICIT_EULER;
 }
 static int bucket_lanes(const gp_batch_opts& o, const Bucket& b) {
-  return b.fast_topo == 100 && o.lanes_per_mech > 0 ? o.lanes_per_mech : 1;
+  return fast_lanes(b.fast_topo, o.lanes_per_mech, o.lanes_per_tree);
 }
 static int group_block_size(const gp_batch_opts& o) { return o.block_size > 0 ? o.block_size : 128; }
 
@@ -467,6 +467,8 @@ static int validate_opts(const gp_batch_opts& o) {
   if (o.lanes_per_mech != 0 && o.lanes_per_mech != 1 && o.lanes_per_mech != 2 && o.lanes_per_mech != 4 &&
       o.lanes_per_mech != 8)
     return fail(GP_ERR_INVALID, "lanes_per_mech must be 1, 2, 4 or 8");
+  if (o.lanes_per_tree != 0 && o.lanes_per_tree != 1 && o.lanes_per_tree != 4)
+    return fail(GP_ERR_INVALID, "lanes_per_tree must be 1 or 4");
   return GP_OK;
 }
 
@@ -774,3 +776,134 @@ int gp_batch_last_timing(const gp_batch* bt, double* ms, int64_t* launches) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ autotuner
+namespace {
+
+// Device copies of every bucket's mutable state, so that trial rollouts leave
+// the batch as they found it.
+struct StateSnapshot {
+  struct Saved { double *q = nullptr, *v = nullptr, *vdot = nullptr; int* fault = nullptr; };
+  std::vector<Saved> saved;
+  ~StateSnapshot() {
+    for (Saved& s : saved) {
+      (void)hipFree(s.q); (void)hipFree(s.v); (void)hipFree(s.vdot); (void)hipFree(s.fault);
+    }
+  }
+  static size_t qn(const Bucket& b) { return (size_t)std::max(1, b.nq) * b.Bp; }
+  static size_t vn(const Bucket& b) { return (size_t)std::max(1, b.nv) * b.Bp; }
+  int take(gp_batch* bt) {
+    saved.resize(bt->buckets.size());
+    for (size_t i = 0; i < bt->buckets.size(); ++i) {
+      const Bucket& b = bt->buckets[i];
+      Saved& s = saved[i];
+      HIP_TRY(hipMalloc((void**)&s.q, qn(b) * sizeof(double)));
+      HIP_TRY(hipMalloc((void**)&s.v, vn(b) * sizeof(double)));
+      HIP_TRY(hipMalloc((void**)&s.vdot, vn(b) * sizeof(double)));
+      HIP_TRY(hipMalloc((void**)&s.fault, (size_t)b.Bp * sizeof(int)));
+    }
+    return copy(bt, true);
+  }
+  int copy(gp_batch* bt, bool save) {
+    for (size_t i = 0; i < bt->buckets.size(); ++i) {
+      const Bucket& b = bt->buckets[i];
+      Saved& s = saved[i];
+      auto cp = [&](void* dev, void* keep, size_t bytes) {
+        return hipMemcpyAsync(save ? keep : dev, save ? dev : keep, bytes, hipMemcpyDeviceToDevice, bt->stream);
+      };
+      HIP_TRY(cp(b.d_q, s.q, qn(b) * sizeof(double)));
+      HIP_TRY(cp(b.d_v, s.v, vn(b) * sizeof(double)));
+      HIP_TRY(cp(b.d_vdot, s.vdot, vn(b) * sizeof(double)));
+      HIP_TRY(cp(b.d_fault, s.fault, (size_t)b.Bp * sizeof(int)));
+    }
+    HIP_TRY(hipStreamSynchronize(bt->stream));
+    return GP_OK;
+  }
+};
+
+// Milliseconds of one trial rollout under the batch's current options (the
+// faster of two runs, each from the saved state).
+int trial_ms(gp_batch* bt, StateSnapshot& snap, double dt, int integrator, int steps, double* ms) {
+  double best = 1e300;
+  for (int rep = 0; rep < 2; ++rep) {
+    int st = snap.copy(bt, false);
+    if (st) return st;
+    if (grouped_count(bt, integrator) > 0 && (st = ensure_group(bt, integrator))) return st;
+    int64_t launches = 0;
+    HIP_TRY(hipEventRecord(bt->ev0, bt->stream));
+    if ((st = batch_launch_all(bt, dt, integrator, steps, bt->stream, &launches))) return st;
+    HIP_TRY(hipEventRecord(bt->ev1, bt->stream));
+    HIP_TRY(hipEventSynchronize(bt->ev1));
+    float f = 0;
+    HIP_TRY(hipEventElapsedTime(&f, bt->ev0, bt->ev1));
+    best = std::min(best, (double)f);
+  }
+  *ms = best;
+  return GP_OK;
+}
+
+}  // namespace
+
+extern "C" int gp_batch_autotune(gp_batch* bt, double dt, int32_t integrator, int32_t trial_steps,
+                                 gp_batch_opts* chosen, double* best_ms) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(GP_ERR_INVALID, "dt must be positive");
+  if (integrator < GP_SEMI_IMPLICIT_EULER || integrator > GP_RUNGE_KUTTA_4)
+    return fail(GP_ERR_INVALID, "unknown integrator %d", integrator);
+  if (trial_steps < 0) return fail(GP_ERR_INVALID, "trial_steps < 0");
+  const int steps = trial_steps > 0 ? trial_steps : 200;
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  bool has_free = false, has_tree = false, has_fast = false;
+  for (const Bucket& b : bt->buckets) {
+    const bool fast = bucket_is_fast(bt->opts, b, integrator);
+    has_fast = has_fast || fast;
+    has_free = has_free || (fast && b.fast_topo == TOPO_FREE);
+    has_tree = has_tree || (fast && b.fast_topo == TOPO_QUAD);
+  }
+  const gp_batch_opts user = bt->opts;
+  StateSnapshot snap;
+  int st = snap.take(bt);
+  if (st) return st;
+  gp_batch_opts best = user;
+  best.use_graph = 0;  // trials time the launches themselves
+  double best_t = 0;
+  auto run = [&](const gp_batch_opts& o, double* t) {
+    int e = gp_batch_set_opts(bt, &o);
+    return e ? e : trial_ms(bt, snap, dt, integrator, steps, t);
+  };
+  st = run(best, &best_t);
+  // coordinate descent over the options that change speed but not the step:
+  // lanes for contact-heavy free bodies, lanes for quadruped trees,
+  // workgroup size, steps fused per launch
+  struct Dim { int32_t gp_batch_opts::*field; std::vector<int> values; bool used; };
+  const Dim dims[] = {
+      {&gp_batch_opts::lanes_per_mech, {1, 2, 4, 8}, has_free},
+      {&gp_batch_opts::lanes_per_tree, {1, 4}, has_tree},
+      {&gp_batch_opts::block_size, {64, 128, 256}, has_fast},
+      {&gp_batch_opts::steps_per_launch, {25, 50, 100, 200}, has_fast && integrator == GP_SEMI_IMPLICIT_EULER},
+  };
+  for (const Dim& d : dims) {
+    if (st || !d.used) continue;
+    for (int v : d.values) {
+      if (best.*d.field == v) continue;
+      gp_batch_opts o = best;
+      o.*d.field = v;
+      double t = 0;
+      if ((st = run(o, &t))) break;
+      if (t < best_t) { best_t = t; best = o; }
+    }
+  }
+  // leave the batch in its saved state with the chosen options (or, after an
+  // error, the caller's)
+  gp_batch_opts fin = best;
+  fin.use_graph = user.use_graph;
+  const int st2 = gp_batch_set_opts(bt, st ? &user : &fin);
+  const int st3 = snap.copy(bt, false);
+  if (st) return st;
+  if (st2) return st2;
+  if (st3) return st3;
+  bt->timing_valid = false;
+  if (chosen) *chosen = fin;
+  if (best_ms) *best_ms = best_t;
+  return GP_OK;
+}

@@ -190,12 +190,13 @@ int rebuild_group(gp_server* s) {
     Pool& p = *pp;
     if (p.bytes == 0) continue;  // released
     if (t.n >= kMaxGroup) return fail(GP_ERR_UNSUPPORTED, "more than %d live pools", kMaxGroup);
-    BucketArgs a = bucket_args(p.b, 1, nullptr);
+    const int lanes = fast_lanes(p.b.fast_topo, s->o.lanes_per_mech, s->o.lanes_per_tree);
+    BucketArgs a = bucket_args(p.b, lanes, nullptr);
     a.dt = p.dt;
     t.args[t.n] = a;
     t.topo[t.n] = p.b.fast_topo;
     t.block_start[t.n] = blocks;
-    blocks += (p.b.B + block - 1) / block;
+    blocks += (int)(((int64_t)p.b.B * lanes + block - 1) / block);
     ++t.n;
   }
   t.block_start[t.n] = blocks;
@@ -422,6 +423,11 @@ int gp_server_create(const gp_server_opts* opts, gp_server** out) {
   if (s->o.block_size <= 0) s->o.block_size = 128;
   if (s->o.block_size % 64 || s->o.block_size > 256) return fail(GP_ERR_INVALID, "block_size must be 64, 128, 192 or 256");
   if (s->o.mem_budget_bytes < 0) return fail(GP_ERR_INVALID, "negative memory budget");
+  if (s->o.lanes_per_mech != 0 && s->o.lanes_per_mech != 1 && s->o.lanes_per_mech != 2 && s->o.lanes_per_mech != 4 &&
+      s->o.lanes_per_mech != 8)
+    return fail(GP_ERR_INVALID, "lanes_per_mech must be 1, 2, 4 or 8");
+  if (s->o.lanes_per_tree != 0 && s->o.lanes_per_tree != 1 && s->o.lanes_per_tree != 4)
+    return fail(GP_ERR_INVALID, "lanes_per_tree must be 1 or 4");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(GP_ERR_DEVICE, "no HIP device");
   size_t free_b = 0, total_b = 0;

@@ -15,11 +15,12 @@ class Server:
     STATS = ("rounds", "env_steps", "busy_s", "admitted", "completed", "pools", "pool_bytes", "budget_bytes")
 
     def __init__(self, precision=abi.GP_PREC_MIXED, chunk_steps=100, slots_per_pool=4096, block_size=0,
-                 mem_budget_bytes=0):
+                 mem_budget_bytes=0, lanes_per_mech=0, lanes_per_tree=0):
         self.lib = abi.load_library()
         o = abi.gp_server_opts()
         o.precision, o.chunk_steps, o.slots_per_pool = precision, chunk_steps, slots_per_pool
         o.block_size, o.mem_budget_bytes = block_size, mem_budget_bytes
+        o.lanes_per_mech, o.lanes_per_tree = lanes_per_mech, lanes_per_tree
         h = C.c_void_p()
         abi.check(self.lib.gp_server_create(C.byref(o), C.byref(h)), self.lib)
         self.handle = h

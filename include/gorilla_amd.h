@@ -127,11 +127,14 @@ typedef struct gp_batch_opts {
   int32_t kernel;            /* GP_KERNEL_*                                   */
   int32_t steps_per_launch;  /* FAST kernels: steps fused in one launch (0 = 1) */
   int32_t block_size;        /* threads per workgroup (0 = 256)               */
-  int32_t lanes_per_mech;    /* lanes cooperating on one instance (0 = 1)     */
+  int32_t lanes_per_mech;    /* FAST free bodies with contacts: lanes sharing
+                                one instance's contact points, 1/2/4/8 (0 = 4) */
   int32_t use_graph;         /* capture multi-launch rollouts in a hipGraph   */
   int32_t separate_launches; /* 1: one launch per topology bucket instead of
                                 one grouped launch over all FAST buckets      */
-  int32_t _reserved;
+  int32_t lanes_per_tree;    /* FAST floating base with 4 three-joint legs:
+                                1 = one lane per instance, 4 = one lane per leg
+                                (0 = 4)                                         */
 } gp_batch_opts;
 
 typedef struct gp_model gp_model;
@@ -214,6 +217,20 @@ int gp_batch_clear_faults(gp_batch* batch);
  * launch): total milliseconds and number of kernel launches. */
 int gp_batch_last_timing(const gp_batch* batch, double* ms, int64_t* launches);
 
+/* Autotuner. No counterpart in the reference (one mechanism per process, no
+ * launch shape to choose). Times short trial rollouts of this batch (trial_steps
+ * steps each, 0 = 200; the faster of two runs) while it walks, one option at a
+ * time, lanes_per_mech (1/2/4/8, if a FAST free-body bucket exists),
+ * lanes_per_tree (1/4, if a FAST quadruped bucket exists), block_size
+ * (64/128/256) and steps_per_launch (25/50/100/200), keeping each value that is
+ * faster. The batch's state, vdot and fault flags are restored afterwards and
+ * the chosen options are applied (use_graph kept as set) and written to
+ * *chosen; *best_ms is the chosen trial's time. Workgroup size and steps per
+ * launch do not change any result bit; the lane counts change the order of
+ * the contact / leg sums, i.e. results at the level of rounding. Blocking. */
+int gp_batch_autotune(gp_batch* batch, double dt, int32_t integrator, int32_t trial_steps,
+                      gp_batch_opts* chosen, double* best_ms);
+
 /* ---- serving ----------------------------------------------------------------
  * No counterpart in the reference, which steps one mechanism in-process
  * (simulate.rs:87). A request is one rollout: a model, its initial state, held
@@ -231,7 +248,9 @@ typedef struct gp_server_opts {
   int32_t slots_per_pool;   /* lanes of one pool (0 = 4096)                   */
   int32_t block_size;       /* threads per workgroup (0 = 128)                */
   int64_t mem_budget_bytes; /* pools' device memory (0 = half the free memory) */
-  int32_t _reserved[4];
+  int32_t lanes_per_mech;   /* as gp_batch_opts (0 = 4)                       */
+  int32_t lanes_per_tree;   /* as gp_batch_opts (0 = 4)                       */
+  int32_t _reserved[2];
 } gp_server_opts;
 typedef struct gp_server gp_server;
 

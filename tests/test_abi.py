@@ -125,3 +125,29 @@ def test_batch_without_device_reports_device_error(gp):
 def test_oracle_library_loads(oracle):
     from gorilla_physics_amd import workloads
     assert oracle.dims(workloads.cube_desc()) == (7, 6)
+
+
+def test_struct_layouts_match_the_header(gp, tmp_path):
+    """The ctypes mirrors have the sizes and field offsets that a C compiler
+    gives the header's structs."""
+    import subprocess
+    structs = ["gp_body_desc", "gp_contact_desc", "gp_halfspace_desc", "gp_model_desc", "gp_batch_opts",
+               "gp_server_opts"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gorilla_amd.h"', 'int main(void) {']
+    for s in structs:
+        py = getattr(gp.abi, s)
+        lines.append(f'  printf("{s} %zu\\n", sizeof({s}));')
+        for f in py._fields_:
+            lines.append(f'  printf("{s}.{f[0]} %zu\\n", offsetof({s}, {f[0]}));')
+    lines.append('  return 0; }')
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.split("\n") if l)
+    for s in structs:
+        py = getattr(gp.abi, s)
+        assert int(got[s]) == C.sizeof(py), s
+        for f in py._fields_:
+            assert int(got[f"{s}.{f[0]}"]) == getattr(py, f[0]).offset, (s, f[0])

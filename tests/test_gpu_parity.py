@@ -146,3 +146,73 @@ def test_lane_independence(gp):
         b1.step(w["dt"], 0, 50)
         q1, v1 = b1.get_state()
         assert np.array_equal(q1, q_all[7 * i:7 * i + 7]) and np.array_equal(v1, v_all[7 * i:7 * i + 7])
+
+
+def _quadrupeds(n, prismatic_knee):
+    """n floating quadrupeds of the mixed workload; with prismatic_knee, the
+    knee of leg 2 of every other instance slides along its axis against a
+    spring (k = 40 N/m, rest 0.05 m), so the tree kernel's per-lane
+    revolute/prismatic selection runs."""
+    from gorilla_physics_amd import abi, workloads
+    w = workloads.mixed(n=n, seed=11, tree_fraction=1.0)
+    if prismatic_knee:
+        q = w["q"].copy()
+        for i, d in enumerate(w["models"]):
+            if i % 2 == 0:
+                d._bodies[9].jtype = abi.GP_JOINT_PRISMATIC
+                d._bodies[9].spring_k, d._bodies[9].spring_l = 40.0, 0.05
+                q[19 * i + 7 + 8] = 0.02 * (i % 5)
+        w["q"] = q
+    return w
+
+
+@pytest.mark.parametrize("prismatic_knee", [False, True])
+@pytest.mark.parametrize("lanes_per_tree", [1, 4])
+def test_quadruped_tree_parity(gp, oracle, lanes_per_tree, prismatic_knee):
+    """Floating quadrupeds through the compiled tree kernels: one lane per
+    instance (lanes_per_tree = 1) and one lane per leg with the legs' base
+    contributions summed across lanes (4). FP64 within 1e-10 of the oracle,
+    mixed within the mixed bounds; 40 steps of the FP64 rollout against the
+    oracle's."""
+    from gorilla_physics_amd import abi
+    w = _quadrupeds(384, prismatic_knee)
+    descs = _descs(w)
+    print(f"case quadruped tree lanes={lanes_per_tree} prismatic={prismatic_knee}", flush=True)
+    for prec in (abi.GP_PREC_F64, abi.GP_PREC_MIXED):
+        q0, v0, q1, v1, a1 = _run_step(gp, w, precision=prec, kernel=abi.GP_KERNEL_FAST,
+                                       lanes_per_tree=lanes_per_tree)
+        eq, ev, ea = step_errors(oracle, descs, q0, v0, w["tau"], q1, v1, a1, w["dt"])
+        print(f"prec={prec}: max e(q') {eq.max():.3g} e(v') {ev.max():.3g} e(qdd) {ea.max():.3g}", flush=True)
+        if prec == abi.GP_PREC_F64:
+            assert eq.max() <= F64_TOL and ev.max() <= F64_TOL and ea.max() <= F64_TOL
+        else:
+            assert eq.max() <= MIXED_Q_TOL and ev.max() <= MIXED_VA_TOL and ea.max() <= MIXED_VA_TOL
+    from gorilla_physics_amd import batch
+    b, _ = batch.batch_from_workload(w, precision=abi.GP_PREC_F64, kernel=abi.GP_KERNEL_FAST,
+                                     lanes_per_tree=lanes_per_tree, steps_per_launch=16)
+    b.step(w["dt"], 0, 40)
+    qg, vg = b.get_state()
+    rq, rv, fails = oracle.rollout(descs, w["q"], w["v"], w["tau"], w["dt"], 40)
+    assert fails == 0 and not b.faults().any()
+    assert rel_err(qg, rq) < 1e-9 and rel_err(vg, rv) < 1e-8
+
+
+def test_quadruped_tree_and_serial_agree_in_a_mix(gp):
+    """The tree kernel inside a grouped launch (quadrupeds beside chains) and
+    in a launch of its own give the same bits; the serial kernel agrees to
+    rounding."""
+    from gorilla_physics_amd import abi, batch, workloads
+    w = workloads.mixed(n=1024, seed=4)
+    print("case quadruped tree grouped vs separate", flush=True)
+    out = {}
+    for name, kw in (("tree_grouped", {}), ("tree_separate", dict(separate_launches=1)),
+                     ("serial", dict(lanes_per_tree=1))):
+        b, _ = batch.batch_from_workload(w, precision=abi.GP_PREC_F64, kernel=abi.GP_KERNEL_FAST, **kw)
+        b.step(w["dt"], 0, 30)
+        out[name] = b.get_state()
+    assert np.array_equal(out["tree_grouped"][0], out["tree_separate"][0], equal_nan=True)
+    assert np.array_equal(out["tree_grouped"][1], out["tree_separate"][1], equal_nan=True)
+    qt, qs = out["tree_grouped"][0], out["serial"][0]
+    fin = np.isfinite(qt)
+    assert np.array_equal(fin, np.isfinite(qs)) and fin.mean() > 0.99
+    assert rel_err(qt[fin], qs[fin]) < 1e-10
