@@ -163,6 +163,17 @@ def serve_baseline_plain(reqs, rate, seed=1):
                 mean_service_ms=float(np.mean(service) * 1e3))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(w, budget_s=12.0):
     """The oracle (CRBA + RNEA + LU, the reference's algorithm) over a bounded
     sample of the same workload, one instance per thread at a time."""
@@ -193,14 +204,15 @@ def cpu_baseline(w, budget_s=12.0):
     o.rollout(descs[:n], w["q"][:qo[n]], w["v"][:vo[n]], w["tau"][:vo[n]], w["dt"], steps, threads=cores)
     sec = time.perf_counter() - t0
     return dict(value=n * steps / sec, unit="env-steps/s", cores=cores, kind="port",
-                sample=f"{n} of {n_all} instances x {steps} steps ({sec:.1f} s), CRBA+RNEA+LU FP64 oracle, "
-                       f"{cores} threads")
+                sample=f"{n} of {n_all} instances x {steps} steps ({sec:.1f} s), CRBA+RNEA+LU FP64 oracle (g++ -O3), "
+                       f"{cores} threads of {os.cpu_count()} on {cpu_model()}")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (0 = the workload's own: arms and mixed "
+                                                            "1000, cubes 5000, acrobots 2000)")
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="arms")
     ap.add_argument("--instances", type=int, default=0)
@@ -246,6 +258,8 @@ def main():
     w = make_workload(args.workload, args.instances)
     models = [batch.Model(d) for d in w["models"]]
     log(f"workload {args.workload}: {sum(w['counts'])} instances")
+    if args.steps <= 0:
+        args.steps = int(w["steps"])
     spl = args.steps_per_launch or min(args.steps, 100)
     common = dict(block_size=args.block, lanes_per_mech=args.lanes, lanes_per_tree=args.lanes_tree,
                   steps_per_launch=spl, use_graph=args.graph, autotune=bool(args.autotune))
@@ -257,8 +271,8 @@ def main():
         f64 = run_variant(w, models, args.steps, args.warmup, precision=abi.GP_PREC_F64, kernel=abi.GP_KERNEL_AUTO,
                           **common)
         log("fast f64", f64)
-        plain_steps = max(10, args.steps // 10)
-        plain = run_variant(w, models, plain_steps, min(args.warmup, 10), precision=abi.GP_PREC_F64,
+        # the same steps (for the cubes, the same free-fall and contact phases)
+        plain = run_variant(w, models, args.steps, min(args.warmup, 10), precision=abi.GP_PREC_F64,
                             kernel=abi.GP_KERNEL_PLAIN, use_graph=args.graph)  # one step per launch: nothing to tune
         log("plain f64", plain)
         extra = dict(fast_f64_env_steps_per_s=f64["value"], plain_f64_env_steps_per_s=plain["value"],

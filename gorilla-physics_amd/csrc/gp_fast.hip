@@ -12,6 +12,8 @@ int launch_fast_f32(int topo, const BucketArgs& a, double dt, int n_steps, int b
 int launch_fast_f64(int topo, const BucketArgs& a, double dt, int n_steps, int block, hipStream_t s);
 int launch_group_f32(const GroupTable* d_table, int n_blocks, double dt, int n_steps, int block, hipStream_t s);
 int launch_group_f64(const GroupTable* d_table, int n_blocks, double dt, int n_steps, int block, hipStream_t s);
+int launch_fast_rk_f32(int topo, const BucketArgs& a, double dt, int n_steps, int integrator, int block, hipStream_t s);
+int launch_fast_rk_f64(int topo, const BucketArgs& a, double dt, int n_steps, int integrator, int block, hipStream_t s);
 
 int fast_topology_id(int nb, const int* parent, const int* dofcls, int nc) {
   bool chain = nb >= 1 && nb <= 12 && nc == 0;
@@ -31,6 +33,15 @@ int launch_fast_step(int topo_id, int precision, const BucketArgs& a, double dt,
   hipStream_t s = (hipStream_t)stream;
   if (precision == GP_PREC_MIXED) return launch_fast_f32(topo_id, a, dt, n_steps, block, s);
   return launch_fast_f64(topo_id, a, dt, n_steps, block, s);
+}
+
+int launch_fast_rk_step(int topo_id, int precision, int integrator, const BucketArgs& a, double dt, int n_steps,
+                        int block, void* stream) {
+  if (a.B == 0 || n_steps <= 0) return 0;
+  if (block <= 0) block = 256;
+  hipStream_t s = (hipStream_t)stream;
+  if (precision == GP_PREC_MIXED) return launch_fast_rk_f32(topo_id, a, dt, n_steps, integrator, block, s);
+  return launch_fast_rk_f64(topo_id, a, dt, n_steps, integrator, block, s);
 }
 
 int launch_grouped_step(const GroupTable* d_table, int n_blocks, int precision, double dt, int n_steps, int block,

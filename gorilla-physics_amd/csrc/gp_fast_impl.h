@@ -16,7 +16,12 @@
 // Register budget: per 1-DOF body only (cos, sin, displacement), the bias
 // acceleration's four non-zero components, p^A, U, D and u are kept across
 // the sweeps; the Plucker transform E = (R_t R_z(q))^T and r are rebuilt from
-// the tree parameters when a sweep needs them.
+// the tree parameters when a sweep needs them. Torques are read where used and
+// accelerations are folded into v and q in the outward pass, so only q and v
+// persist between steps. Every per-bucket array is read and written through a
+// raw buffer resource (one shared 32-bit lane offset instead of a 64-bit
+// address per access), and per-lane revolute/prismatic choices select values,
+// never addresses: with both, no FP32 chain of up to 12 joints spills.
 #include <hip/hip_runtime.h>
 
 #include "../../include/gorilla_amd.h"
@@ -62,6 +67,43 @@ template <class R> __device__ __forceinline__ const R* pick(const double* d, con
 template <> __device__ __forceinline__ const double* pick<double>(const double* d, const float*) { return d; }
 template <> __device__ __forceinline__ const float* pick<float>(const double*, const float* f) { return f; }
 
+// One structure-of-arrays block ([row][Bp lanes]) as a raw buffer resource:
+// a load or store names its row with a uniform byte offset (row * Bp * size,
+// scalar registers) and its lane with one 32-bit VGPR offset shared by every
+// access, instead of a 64-bit per-lane address per access. The hardware range
+// check drops accesses beyond the block (loads return 0). The host keeps every
+// block below 2^31 bytes.
+constexpr int kRawBufferWord3 = 0x00020000;  // CDNA raw buffer: 32-bit data format
+template <class T>
+struct Soa {
+  __amdgpu_buffer_rsrc_t r;
+  int pitch;  // bytes per row
+  __device__ __forceinline__ Soa(const T* base, int rows, int Bp)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0, rows * Bp * (int)sizeof(T),
+                                            kRawBufferWord3)),
+        pitch(Bp * (int)sizeof(T)) {}
+  __device__ __forceinline__ T ld(int row, int lane) const {
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, lane * 8, row * pitch, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, row * pitch, 0));
+  }
+  __device__ __forceinline__ void st(int row, int lane, T x) const {
+    if constexpr (sizeof(T) == 8) {
+      using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), r, lane * 8, row * pitch, 0);
+    } else {
+      using V = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, x), r, lane * 4, row * pitch, 0);
+    }
+  }
+};
+
+// A per-lane choice between two values. Taking them by value keeps the
+// compiler from selecting between two addresses inside a register-resident
+// aggregate, which would force that aggregate into scratch memory.
+template <class T> __device__ __forceinline__ T sel(bool c, T x, T y) { return c ? x : y; }
+
 template <class R, int L> __device__ __forceinline__ R lane_sum(R x) {
 #pragma unroll
   for (int off = 1; off < L; off <<= 1) x += __shfl_xor(x, off, L);
@@ -85,17 +127,31 @@ struct Body1 {  // what a 1-DOF body keeps between the sweeps
   sv<R> pA;        // articulated bias force
   sv<R> U;         // I^A S
   double D, u;     // S^T I^A S and tau - S^T p^A
-  bool rev;
+  int rev;  // 1 revolute, 0 prismatic
 };
 
-template <class Topo, class R>
+// Per-body tree parameters of one instance ([NBF][N][Bp] in the bucket).
+// CACHE = false: read from the bucket's array where used (an L1/L2 hit).
+// CACHE = true: loaded into registers once per launch (loads whose values are
+// never used, e.g. spring constants of an all-revolute chain, are dropped).
+// Caching is right while the ~22 values per body fit beside the sweep state:
+// short chains (see cache_params); longer ones re-read every sweep.
+template <class Topo, class R, bool CACHE = false>
 struct FastParams {
-  const R* bp;
-  int Bp, lane;
-  // uniform column base (scalar unit) + lane offset (one VGPR for all loads)
+  Soa<R> bp;
+  int lane;
+  R c[CACHE ? NBF : 1][CACHE ? Topo::N : 1];
+  __device__ __forceinline__ FastParams(const R* p, int Bp, int lane_) : bp(p, NBF * Topo::N, Bp), lane(lane_) {
+    if constexpr (CACHE) {
+#pragma unroll
+      for (int f = 0; f < NBF; ++f)
+#pragma unroll
+        for (int i = 0; i < Topo::N; ++i) c[f][i] = bp.ld(f * Topo::N + i, lane);
+    }
+  }
   __device__ __forceinline__ R operator()(int f, int i) const {
-    const R* column = bp + (size_t)(f * Topo::N + i) * (size_t)Bp;
-    return column[lane];
+    if constexpr (CACHE) return c[f][i];
+    else return bp.ld(f * Topo::N + i, lane);
   }
   __device__ __forceinline__ m3<R> Rt(int i) const {
     m3<R> M;
@@ -111,14 +167,63 @@ struct FastParams {
   }
 };
 
+// Longest chain whose parameters stay in registers for a whole launch without
+// spilling (FP32 / FP64 sweeps, all-revolute; one body fewer with per-lane
+// joint kinds); a lone free body always keeps its one body's.
+#ifndef GP_PCACHE_F32
+#define GP_PCACHE_F32 9
+#endif
+#ifndef GP_PCACHE_F64
+#define GP_PCACHE_F64 4
+#endif
+template <class Topo, class R, bool ALLREV> constexpr bool cache_params() {
+  constexpr int n_max = (sizeof(R) == 4 ? GP_PCACHE_F32 : GP_PCACHE_F64) - (ALLREV ? 0 : 1);
+  return Topo::N == 1 || (Topo::dof(0) == 1 && Topo::N <= n_max);
+}
+
+// A lone floating body: its articulated inertia is its rigid inertia, the same
+// every step, so its inverse is formed once per launch.
+template <class Topo> constexpr bool lone_free_body() { return Topo::N == 1 && Topo::dof(0) == 6; }
+
+// index of (r, c) in a packed symmetric 6x6 (lower triangle, row-major)
+constexpr __host__ __device__ int sym6(int r, int c) { return r >= c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r; }
+
+// I^-1 (packed symmetric) of a rigid-body inertia; false if I is not SPD.
+template <class R>
+__device__ __forceinline__ bool inverse_inertia(const ai6<R>& I, double (&Iinv)[21]) {
+  double D6[6][6], Lf[21];
+  to_dense(I, D6);
+  const bool ok = chol6(D6, Lf);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double e[6] = {0, 0, 0, 0, 0, 0};
+    e[k] = 1.0;
+    chol6_solve(Lf, e);
+#pragma unroll
+    for (int r = k; r < 6; ++r) Iinv[sym6(r, k)] = e[r];
+  }
+  return ok;
+}
+
 // Articulated-body dynamics of one instance with a compile-time topology.
-// ALLREV: every 1-DOF joint of the bucket is revolute (no per-lane select).
-template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
-__device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int sub, const double* q,
-                                              const double* v, const double* tau, double* vdot) {
+// SIE = true: one semi-implicit Euler step (integrators.rs:276-319), each
+// joint's v and q advanced as soon as its acceleration is known in the outward
+// pass (nothing later in the step reads them), so no acceleration array is
+// held; on the last step the accelerations are stored to vdot. SIE = false:
+// the accelerations at (q, v) go to qdd_out and q, v are left alone (one stage of
+// RK2 / RK4). Torques are read from memory where used. ALLREV: every 1-DOF
+// joint of the bucket is revolute (no per-lane select). Iinv: the inverse
+// inertia of a lone free body (lone_free_body), else unused.
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV, bool SIE, bool PC>
+__device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<Topo, R, PC>& P, int lane, int sub,
+                                          double* q, double* v, double dt, bool last, const double* Iinv,
+                                          double* qdd_out) {
   constexpr int N = Topo::N;
-  const FastParams<Topo, R> P{pick<R>(a.bp, a.bpf), a.Bp, lane};
   const int Bp = a.Bp;
+  const Soa<double> tauS(a.tau, a.nv, Bp), vdotS(a.vdot, a.nv, Bp), bpS(a.bp, NBF * N, Bp);
+  const Soa<int> kindS(a.kind, a.nb, Bp);
+  const Soa<double> cpS(a.cp, NCF * a.nc, Bp), hpS(a.hp, NHF * a.nh, Bp);
+  auto tau = [&](int k) { return tauS.ld(k, lane); };
   Body1<R> b[N];
   sv<R> vel[N];
   // the floating root (if any) keeps its full transform
@@ -139,7 +244,7 @@ __device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int
     m3<double> Rm_d;
     v3<double> p_d;
     if (Topo::dof(i) == 1) {
-      const bool rev = ALLREV || a.kind[i * Bp + lane] == K_REV;
+      const bool rev = ALLREV || kindS.ld(i, lane) == K_REV;
       b[i].rev = rev;
       const double qi = q[qoff_of<Topo>(i)];
       sincos_q(rev ? qi : 0.0, &b[i].s, &b[i].c);
@@ -160,9 +265,9 @@ __device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int
       m3<double> Rtd;
       v3<double> ptd;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Rtd.a[k] = a.bp[(size_t)((BF_RT + k) * N + i) * Bp + lane];
+      for (int k = 0; k < 9; ++k) Rtd.a[k] = bpS.ld((BF_RT + k) * N + i, lane);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) (&ptd.x)[k] = a.bp[(size_t)((BF_PT + k) * N + i) * Bp + lane];
+      for (int k = 0; k < 3; ++k) (&ptd.x)[k] = bpS.ld((BF_PT + k) * N + i, lane);
       Rm_d = mul(Rtd, Rq);
       p_d = ptd + mul(Rtd, tq);
       const m3<R> Rm = cvt<R>(Rm_d);
@@ -178,12 +283,12 @@ __device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int
     ai6<R> I = P.inertia(i);
     sv<R> pA = crf(V, mul(I, V));
     if (Topo::dof(i) == 1) {
-      const R qd = b[i].rev ? vj.w.z : vj.v.z;
+      const R qd = sel(b[i].rev, vj.w.z, vj.v.z);
       // v x_m (S qd) with S = e_z angular (revolute) or linear (prismatic)
       b[i].cw0 = b[i].rev ? qd * V.w.y : R(0);
       b[i].cw1 = b[i].rev ? -qd * V.w.x : R(0);
-      b[i].cv0 = qd * (b[i].rev ? V.v.y : V.w.y);
-      b[i].cv1 = -qd * (b[i].rev ? V.v.x : V.w.x);
+      b[i].cv0 = qd * sel(b[i].rev, V.v.y, V.w.y);
+      b[i].cv1 = -qd * sel(b[i].rev, V.v.x, V.w.x);
     } else {
       cb0 = crm(V, vj);
     }
@@ -194,14 +299,14 @@ __device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int
       sv<R> fe{{R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
       const int c0 = a.cstart[i], c1 = a.cstart[i + 1];
       for (int cc = c0 + sub; cc < c1; cc += L) {
-        auto cpar = [&](int fld) { return a.cp[((size_t)fld * a.nc + cc) * Bp + lane]; };
+        auto cpar = [&](int fld) { return cpS.ld(fld * a.nc + cc, lane); };
         v3<double> pb{cpar(CF_P), cpar(CF_P + 1), cpar(CF_P + 2)};
         const R kc = R(cpar(CF_K));
         v3<double> pw = mul(Rw[i], pb) + tw[i];
         v3<R> pbR = cvt<R>(pb);
         v3<R> vw = mul(RwR, V.v + cross(V.w, pbR));
         for (int h = 0; h < a.nh; ++h) {
-          auto hpar = [&](int fld) { return a.hp[((size_t)fld * a.nh + h) * Bp + lane]; };
+          auto hpar = [&](int fld) { return hpS.ld(fld * a.nh + h, lane); };
           v3<double> n{hpar(HF_NORMAL), hpar(HF_NORMAL + 1), hpar(HF_NORMAL + 2)};
           double dist = dot(pw - v3<double>{hpar(HF_POINT), hpar(HF_POINT + 1), hpar(HF_POINT + 2)}, n);
           if (dist <= kInsideMargin) {
@@ -235,13 +340,14 @@ __device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int
     if (Topo::dof(i) == 1) {
       Body1<R>& B = b[i];
       B.U = B.rev ? col(IA, 2) : col(IA, 5);
-      B.D = double(B.rev ? IA.A.zz : IA.C.zz);
-      double tq = tau[voff_of<Topo>(i)];
-      if (!B.rev) {
-        const R sk = P(BF_SK, i);
-        if (sk != R(0)) tq += -double(sk) * (q[qoff_of<Topo>(i)] - double(P(BF_SL, i)));
+      B.D = double(sel(B.rev, IA.A.zz, IA.C.zz));
+      double tq = tau(voff_of<Topo>(i));
+      if (!ALLREV) {
+        // prismatic spring, branch-free: a revolute joint or k = 0 adds -0
+        const double sk = B.rev ? 0.0 : double(P(BF_SK, i));
+        tq += -sk * (q[qoff_of<Topo>(i)] - double(P(BF_SL, i)));
       }
-      B.u = tq - double(B.rev ? B.pA.w.z : B.pA.v.z);
+      B.u = tq - double(sel(B.rev, B.pA.w.z, B.pA.v.z));
       ok = ok && (B.D > 0.0);
       if (pr >= 0) {
         const R inv = R(1.0 / B.D);
@@ -261,11 +367,13 @@ __device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int
       }
     } else {
       // floating joint (the root in every compiled topology): D = I^A (6x6)
-      to_dense(IA, D6);
       const int w = voff_of<Topo>(i);
-      u6[0] = tau[w] - double(pA0.w.x); u6[1] = tau[w + 1] - double(pA0.w.y); u6[2] = tau[w + 2] - double(pA0.w.z);
-      u6[3] = tau[w + 3] - double(pA0.v.x); u6[4] = tau[w + 4] - double(pA0.v.y); u6[5] = tau[w + 5] - double(pA0.v.z);
-      ok = chol6(D6, Lf) && ok;
+      u6[0] = tau(w) - double(pA0.w.x); u6[1] = tau(w + 1) - double(pA0.w.y); u6[2] = tau(w + 2) - double(pA0.w.z);
+      u6[3] = tau(w + 3) - double(pA0.v.x); u6[4] = tau(w + 4) - double(pA0.v.y); u6[5] = tau(w + 5) - double(pA0.v.z);
+      if constexpr (!lone_free_body<Topo>()) {
+        to_dense(IA, D6);
+        ok = chol6(D6, Lf) && ok;
+      }
     }
   }
   // ---- pass 3: accelerations (root to leaves)
@@ -283,24 +391,53 @@ __device__ __forceinline__ bool fast_dynamics(const BucketArgs& a, int lane, int
       sv<R> ap = xmotion(E, r, pr < 0 ? a0 : accel[pp]);
       ap.w.x += B.cw0; ap.w.y += B.cw1; ap.v.x += B.cv0; ap.v.y += B.cv1;
       const double qdd = (B.u - double(dot6(B.U, ap))) / B.D;
-      vdot[voff_of<Topo>(i)] = qdd;
-      if (B.rev) ap.w.z += R(qdd); else ap.v.z += R(qdd);
+      const int vo = voff_of<Topo>(i);
+      ok = ok && isfinite(qdd);
+      if constexpr (SIE) {
+        if (last && sub == 0) vdotS.st(vo, lane, qdd);
+        v[vo] = v[vo] + qdd * dt;
+        q[qoff_of<Topo>(i)] += v[vo] * dt;
+      } else {
+        qdd_out[vo] = qdd;
+      }
+      ap.w.z = B.rev ? ap.w.z + R(qdd) : ap.w.z;
+      ap.v.z = B.rev ? ap.v.z : ap.v.z + R(qdd);
       accel[i] = ap;
     } else {
       sv<R> ap = xmotion(E0, r0, pr < 0 ? a0 : accel[pp]) + cb0;
       const int w = voff_of<Topo>(i);
       const double apd[6] = {double(ap.w.x), double(ap.w.y), double(ap.w.z), double(ap.v.x), double(ap.v.y), double(ap.v.z)};
       double x[6];
+      if constexpr (lone_free_body<Topo>()) {
+        // x = I^-1 (u - I a_p) = I^-1 u - a_p
 #pragma unroll
-      for (int rr = 0; rr < 6; ++rr) {
-        double s = u6[rr];
+        for (int rr = 0; rr < 6; ++rr) {
+          double s = 0.0;
 #pragma unroll
-        for (int cc = 0; cc < 6; ++cc) s -= D6[rr][cc] * apd[cc];
-        x[rr] = s;
+          for (int cc = 0; cc < 6; ++cc) s += Iinv[sym6(rr, cc)] * u6[cc];
+          x[rr] = s - apd[rr];
+        }
+      } else {
+#pragma unroll
+        for (int rr = 0; rr < 6; ++rr) {
+          double s = u6[rr];
+#pragma unroll
+          for (int cc = 0; cc < 6; ++cc) s -= D6[rr][cc] * apd[cc];
+          x[rr] = s;
+        }
+        chol6_solve(Lf, x);
       }
-      chol6_solve(Lf, x);
 #pragma unroll
-      for (int t = 0; t < 6; ++t) vdot[w + t] = x[t];
+      for (int t = 0; t < 6; ++t) {
+        ok = ok && isfinite(x[t]);
+        if constexpr (SIE) {
+          if (last && sub == 0) vdotS.st(w + t, lane, x[t]);
+          v[w + t] = v[w + t] + x[t] * dt;
+        } else {
+          qdd_out[w + t] = x[t];
+        }
+      }
+      if constexpr (SIE) advance_floating(&q[qoff_of<Topo>(i)], &v[w], dt, &q[qoff_of<Topo>(i)]);
       accel[i] = ap + sv<R>{{R(x[0]), R(x[1]), R(x[2])}, {R(x[3]), R(x[4]), R(x[5])}};
     }
   }
@@ -326,37 +463,30 @@ __device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, doub
     my_steps = min(my_steps, left);
     if (sub == 0) a.active[inst] = left - my_steps;
   }
-  double q[NQ], v[NV], tau[NV], vd[NV];
+  const Soa<double> qS(a.q, NQ, Bp), vS(a.v, NV, Bp);
+  double q[NQ], v[NV];
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) q[k] = a.q[(size_t)k * Bp + inst];
+  for (int k = 0; k < NQ; ++k) q[k] = qS.ld(k, inst);
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    v[k] = a.v[(size_t)k * Bp + inst];
-    tau[k] = a.tau[(size_t)k * Bp + inst];
-    vd[k] = a.vdot[(size_t)k * Bp + inst];
-  }
+  for (int k = 0; k < NV; ++k) v[k] = vS.ld(k, inst);
   bool ok = true;
-  for (int s = 0; s < my_steps; ++s) {
-    ok = fast_dynamics<Topo, R, L, CONTACTS, ALLREV>(a, inst, sub, q, v, tau, vd) && ok;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = v[k] + vd[k] * dt;
-#pragma unroll
-    for (int i = 0; i < Topo::N; ++i) {
-      if (Topo::dof(i) == 1) q[qoff_of<Topo>(i)] += v[voff_of<Topo>(i)] * dt;
-      else advance_floating(&q[qoff_of<Topo>(i)], &v[voff_of<Topo>(i)], dt, &q[qoff_of<Topo>(i)]);
-    }
-  }
+  constexpr bool PC = cache_params<Topo, R, ALLREV>();
+  const FastParams<Topo, R, PC> P(pick<R>(a.bp, a.bpf), Bp, inst);
+  double Iinv[lone_free_body<Topo>() ? 21 : 1];
+  if constexpr (lone_free_body<Topo>()) ok = inverse_inertia(P.inertia(0), Iinv);
+  for (int s = 0; s < my_steps; ++s)
+    ok = fast_step<Topo, R, L, CONTACTS, ALLREV, true, PC>(a, P, inst, sub, q, v, dt, s + 1 == my_steps, Iinv,
+                                                          nullptr) && ok;
   if (sub != 0) return;
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
-    a.q[(size_t)k * Bp + inst] = q[k];
+    qS.st(k, inst, q[k]);
     ok = ok && isfinite(q[k]);
   }
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    a.v[(size_t)k * Bp + inst] = v[k];
-    a.vdot[(size_t)k * Bp + inst] = vd[k];
-    ok = ok && isfinite(v[k]) && isfinite(vd[k]);
+    vS.st(k, inst, v[k]);
+    ok = ok && isfinite(v[k]);
   }
   if (!ok) a.fault[inst] = 1;
 }
@@ -364,6 +494,100 @@ __device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, doub
 template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
 __global__ void __launch_bounds__(256) fast_sie_kernel(BucketArgs a, double dt, int n_steps) {
   fast_sie_body<Topo, R, L, CONTACTS, ALLREV>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps);
+}
+
+// euler_step (integrators.rs:230-271): qo = q (+) h v, vo = v + h acc. In place
+// (qo = q, vo = v) is allowed: q is advanced before v changes.
+template <class Topo>
+__device__ __forceinline__ void euler_stage(const double* q, const double* v, const double* acc, double h, double* qo,
+                                            double* vo) {
+#pragma unroll
+  for (int i = 0; i < Topo::N; ++i) {
+    if (Topo::dof(i) == 1) qo[qoff_of<Topo>(i)] = q[qoff_of<Topo>(i)] + v[voff_of<Topo>(i)] * h;
+    else advance_floating(&q[qoff_of<Topo>(i)], &v[voff_of<Topo>(i)], h, &qo[qoff_of<Topo>(i)]);
+  }
+#pragma unroll
+  for (int k = 0; k < nv_of<Topo>(); ++k) vo[k] = v[k] + acc[k] * h;
+}
+
+// n_steps RK2 or RK4 steps per launch (integrators.rs:177-225) with the state
+// in registers. The reference's stage rule: every stage state is an
+// euler_step from the step's start (q, v) with the previous stage's
+// accelerations (h = dt/2, dt/2, dt), and RK4 combines
+// (a1 + 2 a2 + 2 a3 + a4) / 6 in that order. One copy of the dynamics serves
+// every stage (a stage loop that is not unrolled). vdot receives the last
+// step's first-stage accelerations, as the plain kernel leaves it.
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
+__device__ __forceinline__ void fast_rk_body(const BucketArgs& a, int tid, double dt, int n_steps, int integrator) {
+  constexpr int NQ = nq_of<Topo>(), NV = nv_of<Topo>();
+  const int inst = tid / L;
+  const int sub = tid % L;
+  if (inst >= a.B) return;
+  const int Bp = a.Bp;
+  int my_steps = n_steps;
+  if (a.active) {
+    const int left = a.active[inst];
+    if (left <= 0) return;
+    my_steps = min(my_steps, left);
+    if (sub == 0) a.active[inst] = left - my_steps;
+  }
+  const Soa<double> qS(a.q, NQ, Bp), vS(a.v, NV, Bp), vdotS(a.vdot, NV, Bp);
+  double q[NQ], v[NV];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) q[k] = qS.ld(k, inst);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = vS.ld(k, inst);
+  bool ok = true;
+  constexpr bool PC = cache_params<Topo, R, ALLREV>();
+  const FastParams<Topo, R, PC> P(pick<R>(a.bp, a.bpf), Bp, inst);
+  double Iinv[lone_free_body<Topo>() ? 21 : 1];
+  if constexpr (lone_free_body<Topo>()) ok = inverse_inertia(P.inertia(0), Iinv);
+  const bool rk4 = integrator == GP_RUNGE_KUTTA_4;
+  const int stages = rk4 ? 4 : 2;
+  double qs[NQ], vs[NV], ak[NV], sum[NV];
+  for (int s = 0; s < my_steps; ++s) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) qs[k] = q[k];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) vs[k] = v[k];
+#pragma unroll 1
+    for (int st = 0; st < stages; ++st) {
+      ok = fast_step<Topo, R, L, CONTACTS, ALLREV, false, PC>(a, P, inst, sub, qs, vs, 0.0, false, Iinv, ak) && ok;
+      if (st == 0) {
+        if (s + 1 == my_steps && sub == 0) {
+#pragma unroll
+          for (int k = 0; k < NV; ++k) vdotS.st(k, inst, ak[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sum[k] = ak[k];
+      } else if (st < 3) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sum[k] = sum[k] + ak[k] * 2.0;
+      } else {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sum[k] = (sum[k] + ak[k]) / 6.0;
+      }
+      if (st + 1 < stages) euler_stage<Topo>(q, v, ak, st == 2 ? dt : dt / 2.0, qs, vs);
+      else euler_stage<Topo>(q, v, rk4 ? sum : ak, dt, q, v);
+    }
+  }
+  if (sub != 0) return;
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    qS.st(k, inst, q[k]);
+    ok = ok && isfinite(q[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    vS.st(k, inst, v[k]);
+    ok = ok && isfinite(v[k]);
+  }
+  if (!ok) a.fault[inst] = 1;
+}
+
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
+__global__ void __launch_bounds__(256) fast_rk_kernel(BucketArgs a, double dt, int n_steps, int integrator) {
+  fast_rk_body<Topo, R, L, CONTACTS, ALLREV>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps, integrator);
 }
 
 // ---------------------------------------------------------------- trees, one lane per leg
@@ -410,25 +634,28 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
     my_steps = min(my_steps, left);
     if (leg == 0) a.active[inst] = left - my_steps;
   }
-  const FastParams<TopoTree<LEGS, DEPTH>, R> P{pick<R>(a.bp, a.bpf), Bp, inst};
+  const FastParams<TopoTree<LEGS, DEPTH>, R> P(pick<R>(a.bp, a.bpf), Bp, inst);
+  const Soa<double> qS(a.q, 7 + LEGS * DEPTH, Bp), vS(a.v, 6 + LEGS * DEPTH, Bp), tauS(a.tau, 6 + LEGS * DEPTH, Bp),
+      vdotS(a.vdot, 6 + LEGS * DEPTH, Bp), bpS(a.bp, NBF * N, Bp);
+  const Soa<int> kindS(a.kind, N, Bp);
   // state: base (q 7, v 6) in every lane, this leg's joints
   double qb[7], vb[6], tb[6], ab[6];
   double ql[DEPTH], vl[DEPTH], tl[DEPTH], al[DEPTH];
 #pragma unroll
-  for (int k = 0; k < 7; ++k) qb[k] = a.q[(size_t)k * Bp + inst];
+  for (int k = 0; k < 7; ++k) qb[k] = qS.ld(k, inst);
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    vb[k] = a.v[(size_t)k * Bp + inst];
-    tb[k] = a.tau[(size_t)k * Bp + inst];
-    ab[k] = a.vdot[(size_t)k * Bp + inst];
+    vb[k] = vS.ld(k, inst);
+    tb[k] = tauS.ld(k, inst);
+    ab[k] = vdotS.ld(k, inst);
   }
 #pragma unroll
   for (int k = 0; k < DEPTH; ++k) {
     const int j = leg * DEPTH + k;
-    ql[k] = a.q[(size_t)(7 + j) * Bp + inst];
-    vl[k] = a.v[(size_t)(6 + j) * Bp + inst];
-    tl[k] = a.tau[(size_t)(6 + j) * Bp + inst];
-    al[k] = a.vdot[(size_t)(6 + j) * Bp + inst];
+    ql[k] = qS.ld((7 + j), inst);
+    vl[k] = vS.ld((6 + j), inst);
+    tl[k] = tauS.ld((6 + j), inst);
+    al[k] = vdotS.ld((6 + j), inst);
   }
   bool ok = true;
   for (int s = 0; s < my_steps; ++s) {
@@ -436,9 +663,9 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
     m3<double> Rq = quat_mat(qb[0], qb[1], qb[2], qb[3]);
     m3<double> Rtd;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Rtd.a[k] = a.bp[(size_t)((BF_RT + k) * N) * Bp + inst];
-    v3<double> ptd{a.bp[(size_t)(BF_PT * N) * Bp + inst], a.bp[(size_t)((BF_PT + 1) * N) * Bp + inst],
-                   a.bp[(size_t)((BF_PT + 2) * N) * Bp + inst]};
+    for (int k = 0; k < 9; ++k) Rtd.a[k] = bpS.ld(((BF_RT + k) * N), inst);
+    v3<double> ptd{bpS.ld((BF_PT * N), inst), bpS.ld(((BF_PT + 1) * N), inst),
+                   bpS.ld(((BF_PT + 2) * N), inst)};
     const m3<R> Rm = cvt<R>(mul(Rtd, Rq));
     const m3<R> E0{{Rm.a[0], Rm.a[3], Rm.a[6], Rm.a[1], Rm.a[4], Rm.a[7], Rm.a[2], Rm.a[5], Rm.a[8]}};
     const v3<R> r0 = cvt<R>(ptd + mul(Rtd, v3<double>{qb[4], qb[5], qb[6]}));
@@ -451,7 +678,7 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
 #pragma unroll
     for (int k = 0; k < DEPTH; ++k) {
       const int i = 1 + leg * DEPTH + k;
-      const bool rev = ALLREV || a.kind[i * Bp + inst] == K_REV;
+      const bool rev = ALLREV || kindS.ld(i, inst) == K_REV;
       b[k].rev = rev;
       sincos_q(rev ? ql[k] : 0.0, &b[k].s, &b[k].c);
       b[k].d = rev ? R(0) : R(ql[k]);
@@ -464,8 +691,8 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
       b[k].pA = crf(vel, mul(P.inertia(i), vel));
       b[k].cw0 = rev ? qd * vel.w.y : R(0);
       b[k].cw1 = rev ? -qd * vel.w.x : R(0);
-      b[k].cv0 = qd * (rev ? vel.v.y : vel.w.y);
-      b[k].cv1 = -qd * (rev ? vel.v.x : vel.w.x);
+      b[k].cv0 = qd * sel(rev, vel.v.y, vel.w.y);
+      b[k].cv1 = -qd * sel(rev, vel.v.x, vel.w.x);
     }
     // ---- pass 2: this leg, leaves to its first joint, then the base
     ai6<R> acc;
@@ -478,13 +705,14 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
       ai6<R> IA = P.inertia(i);
       if (k < DEPTH - 1) add_to(IA, acc);
       B.U = B.rev ? col(IA, 2) : col(IA, 5);
-      B.D = double(B.rev ? IA.A.zz : IA.C.zz);
+      B.D = double(sel(B.rev, IA.A.zz, IA.C.zz));
       double tq = tl[k];
-      if (!B.rev) {
-        const R sk = P(BF_SK, i);
-        if (sk != R(0)) tq += -double(sk) * (ql[k] - double(P(BF_SL, i)));
+      if (!ALLREV) {
+        // prismatic spring, branch-free: a revolute joint or k = 0 adds -0
+        const double sk = B.rev ? 0.0 : double(P(BF_SK, i));
+        tq += -sk * (ql[k] - double(P(BF_SL, i)));
       }
-      B.u = tq - double(B.rev ? B.pA.w.z : B.pA.v.z);
+      B.u = tq - double(sel(B.rev, B.pA.w.z, B.pA.v.z));
       ok = ok && (B.D > 0.0);
       const R inv = R(1.0 / B.D);
       const sv<R> uD{inv * B.U.w, inv * B.U.v};
@@ -540,7 +768,8 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
       ap.w.x += B.cw0; ap.w.y += B.cw1; ap.v.x += B.cv0; ap.v.y += B.cv1;
       const double qdd = (B.u - double(dot6(B.U, ap))) / B.D;
       al[k] = qdd;
-      if (B.rev) ap.w.z += R(qdd); else ap.v.z += R(qdd);
+      ap.w.z = B.rev ? ap.w.z + R(qdd) : ap.w.z;
+      ap.v.z = B.rev ? ap.v.z : ap.v.z + R(qdd);
       accel = ap;
     }
     // ---- semi-implicit Euler (integrators.rs:276-319)
@@ -555,20 +784,20 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
   }
   if (leg == 0) {
 #pragma unroll
-    for (int k = 0; k < 7; ++k) { a.q[(size_t)k * Bp + inst] = qb[k]; ok = ok && isfinite(qb[k]); }
+    for (int k = 0; k < 7; ++k) { qS.st(k, inst, qb[k]); ok = ok && isfinite(qb[k]); }
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
-      a.v[(size_t)k * Bp + inst] = vb[k];
-      a.vdot[(size_t)k * Bp + inst] = ab[k];
+      vS.st(k, inst, vb[k]);
+      vdotS.st(k, inst, ab[k]);
       ok = ok && isfinite(vb[k]) && isfinite(ab[k]);
     }
   }
 #pragma unroll
   for (int k = 0; k < DEPTH; ++k) {
     const int j = leg * DEPTH + k;
-    a.q[(size_t)(7 + j) * Bp + inst] = ql[k];
-    a.v[(size_t)(6 + j) * Bp + inst] = vl[k];
-    a.vdot[(size_t)(6 + j) * Bp + inst] = al[k];
+    qS.st((7 + j), inst, ql[k]);
+    vS.st((6 + j), inst, vl[k]);
+    vdotS.st((6 + j), inst, al[k]);
     ok = ok && isfinite(ql[k]) && isfinite(vl[k]) && isfinite(al[k]);
   }
   if (!ok) a.fault[inst] = 1;
@@ -648,6 +877,41 @@ static int launch_tree(const BucketArgs& a, double dt, int n_steps, int block, h
   if (a.all_rev) hipLaunchKernelGGL((tree_sie_kernel<R, LEGS, DEPTH, true>), grid, blk, 0, s, a, dt, n_steps);
   else hipLaunchKernelGGL((tree_sie_kernel<R, LEGS, DEPTH, false>), grid, blk, 0, s, a, dt, n_steps);
   return (int)hipGetLastError();
+}
+
+template <class Topo, class R, int L, bool C, bool AR>
+static int launch_rk_t(const BucketArgs& a, double dt, int n_steps, int integrator, int block, hipStream_t s) {
+  long threads = (long)a.B * L;
+  dim3 blk(block), grid((unsigned)((threads + block - 1) / block));
+  hipLaunchKernelGGL((fast_rk_kernel<Topo, R, L, C, AR>), grid, blk, 0, s, a, dt, n_steps, integrator);
+  return (int)hipGetLastError();
+}
+
+template <class Topo, class R>
+static int launch_rk_chain(const BucketArgs& a, double dt, int n, int integ, int block, hipStream_t s) {
+  if (a.all_rev) return launch_rk_t<Topo, R, 1, false, true>(a, dt, n, integ, block, s);
+  return launch_rk_t<Topo, R, 1, false, false>(a, dt, n, integ, block, s);
+}
+
+// RK2 / RK4 launches (one bucket per launch; quadrupeds one lane per instance)
+template <class R>
+static int launch_rk_prec(int topo, const BucketArgs& a, double dt, int n, int integ, int block, hipStream_t s) {
+  switch (topo) {
+#define GP_RK_CASE(K) \
+    case K: return launch_rk_chain<TopoChain<K>, R>(a, dt, n, integ, block, s);
+    GP_RK_CASE(1) GP_RK_CASE(2) GP_RK_CASE(3) GP_RK_CASE(4) GP_RK_CASE(5) GP_RK_CASE(6)
+    GP_RK_CASE(7) GP_RK_CASE(8) GP_RK_CASE(9) GP_RK_CASE(10) GP_RK_CASE(11) GP_RK_CASE(12)
+#undef GP_RK_CASE
+    case TOPO_QUAD: return launch_rk_chain<TopoQuad, R>(a, dt, n, integ, block, s);
+    case TOPO_FREE:
+      switch (a.lanes) {
+        case 2: return launch_rk_t<TopoFree, R, 2, true, false>(a, dt, n, integ, block, s);
+        case 4: return launch_rk_t<TopoFree, R, 4, true, false>(a, dt, n, integ, block, s);
+        case 8: return launch_rk_t<TopoFree, R, 8, true, false>(a, dt, n, integ, block, s);
+        default: return launch_rk_t<TopoFree, R, 1, true, false>(a, dt, n, integ, block, s);
+      }
+    default: return (int)hipErrorInvalidValue;
+  }
 }
 
 template <class R>

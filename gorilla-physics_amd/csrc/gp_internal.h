@@ -118,6 +118,10 @@ int launch_plain_step(const BucketArgs& a, double dt, int integrator, int dynami
 int fast_topology_id(int nb, const int* parent, const int* dofcls, int nc);
 int launch_fast_step(int topo_id, int precision, const BucketArgs& a, double dt, int n_steps, int block,
                      void* stream);
+// RK2 / RK4 (integrator GP_RUNGE_KUTTA_*) on the fast kernels; one lane per
+// quadruped instance (a.lanes is used for free bodies only).
+int launch_fast_rk_step(int topo_id, int precision, int integrator, const BucketArgs& a, double dt, int n_steps,
+                        int block, void* stream);
 size_t plain_scratch_doubles(int nb, int nq, int nv, int Bp);
 
 }  // namespace gp

@@ -320,6 +320,12 @@ int bucket_ensure_scratch(Bucket& b) {
   return dalloc(&b.d_scratch, plain_scratch_doubles(b.nb, b.nq, b.nv, b.Bp));
 }
 
+int64_t soa_max_lanes(const gp_model& m) {
+  const int64_t per_lane = 8 * std::max<int64_t>({(int64_t)NBF * m.n, (int64_t)NCF * m.nc, (int64_t)NHF * m.nh,
+                                                   (int64_t)m.nq, (int64_t)m.nv, 1});
+  return std::max<int64_t>(64, ((((int64_t)1 << 31) - 1) / per_lane) / 64 * 64);
+}
+
 BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
   BucketArgs a{};
   a.B = b.B; a.Bp = b.Bp; a.nb = b.nb; a.nq = b.nq; a.nv = b.nv; a.nc = b.nc; a.nh = b.nh;
@@ -337,17 +343,19 @@ using namespace gp;
 
 // ------------------------------------------------------------------ batch
 static bool bucket_is_fast(const gp_batch_opts& o, const Bucket& b, int integrator) {
-  return o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0 && integrator == GP_SEMI_IMPLICIT_EULER;
+  return o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0;
 }
-static int bucket_lanes(const gp_batch_opts& o, const Bucket& b) {
-  return fast_lanes(b.fast_topo, o.lanes_per_mech, o.lanes_per_tree);
+static int bucket_lanes(const gp_batch_opts& o, const Bucket& b, int integrator) {
+  // the tree-parallel quadruped kernel is semi-implicit Euler only
+  const int tree = integrator == GP_SEMI_IMPLICIT_EULER ? o.lanes_per_tree : 1;
+  return fast_lanes(b.fast_topo, o.lanes_per_mech, tree);
 }
 static int group_block_size(const gp_batch_opts& o) { return o.block_size > 0 ? o.block_size : 128; }
 
 // Number of FAST buckets that one grouped launch would cover (0 = no grouping).
 static int grouped_count(const gp_batch* bt, int integrator) {
   const gp_batch_opts& o = bt->opts;
-  if (o.separate_launches) return 0;
+  if (o.separate_launches || integrator != GP_SEMI_IMPLICIT_EULER) return 0;
   int n = 0;
   for (const Bucket& b : bt->buckets) n += bucket_is_fast(o, b, integrator) ? 1 : 0;
   return (n >= 2 && n <= kMaxGroup) ? n : 0;
@@ -363,7 +371,7 @@ static int ensure_group(gp_batch* bt, int integrator) {
   int blocks = 0;
   for (const Bucket& b : bt->buckets) {
     if (!bucket_is_fast(o, b, integrator)) continue;
-    const int lanes = bucket_lanes(o, b);
+    const int lanes = bucket_lanes(o, b, integrator);
     t.args[t.n] = bucket_args(b, lanes, nullptr);
     t.topo[t.n] = b.fast_topo;
     t.block_start[t.n] = blocks;
@@ -402,11 +410,13 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
     if (fast && grouped) continue;
     if (fast) {
       int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
-      int lanes = bucket_lanes(o, b);
+      int lanes = bucket_lanes(o, b, integrator);
       BucketArgs a = bucket_args(b, lanes, nullptr);
       for (int done = 0; done < n_steps; done += spl) {
         int n = std::min(spl, n_steps - done);
-        int e = launch_fast_step(b.fast_topo, o.precision, a, dt, n, o.block_size, s);
+        int e = integrator == GP_SEMI_IMPLICIT_EULER
+                    ? launch_fast_step(b.fast_topo, o.precision, a, dt, n, o.block_size, s)
+                    : launch_fast_rk_step(b.fast_topo, o.precision, integrator, a, dt, n, o.block_size, s);
         if (e) return fail(GP_ERR_DEVICE, "fast kernel launch: %s", hipGetErrorString((hipError_t)e));
         ++*launches;
       }
@@ -490,17 +500,19 @@ int gp_batch_create(gp_model* const* models, const int64_t* counts, int32_t n_mo
     for (int k = 0; k < n_models; ++k) {
       if (!models[k]) return fail(GP_ERR_INVALID, "model %d is null", k);
       if (counts[k] < 0) return fail(GP_ERR_INVALID, "count %d is negative", k);
-      auto it = sig_bucket.find(models[k]->sig);
-      int bi;
-      if (it == sig_bucket.end()) {
-        bi = (int)lanes.size();
-        sig_bucket[models[k]->sig] = bi;
-        lanes.emplace_back();
-        inst_of_lane.emplace_back();
-      } else {
-        bi = it->second;
-      }
+      const int64_t cap = soa_max_lanes(*models[k]);
       for (int64_t c = 0; c < counts[k]; ++c) {
+        // a full bucket (its largest array would reach 2^31 bytes) starts another of the same signature
+        auto it = sig_bucket.find(models[k]->sig);
+        int bi;
+        if (it == sig_bucket.end() || (int64_t)lanes[it->second].size() >= cap) {
+          bi = (int)lanes.size();
+          sig_bucket[models[k]->sig] = bi;
+          lanes.emplace_back();
+          inst_of_lane.emplace_back();
+        } else {
+          bi = it->second;
+        }
         lanes[bi].push_back(models[k]);
         inst_of_lane[bi].push_back(inst++);
       }
@@ -878,9 +890,9 @@ extern "C" int gp_batch_autotune(gp_batch* bt, double dt, int32_t integrator, in
   struct Dim { int32_t gp_batch_opts::*field; std::vector<int> values; bool used; };
   const Dim dims[] = {
       {&gp_batch_opts::lanes_per_mech, {1, 2, 4, 8}, has_free},
-      {&gp_batch_opts::lanes_per_tree, {1, 4}, has_tree},
+      {&gp_batch_opts::lanes_per_tree, {1, 4}, has_tree && integrator == GP_SEMI_IMPLICIT_EULER},
       {&gp_batch_opts::block_size, {64, 128, 256}, has_fast},
-      {&gp_batch_opts::steps_per_launch, {25, 50, 100, 200}, has_fast && integrator == GP_SEMI_IMPLICIT_EULER},
+      {&gp_batch_opts::steps_per_launch, {25, 50, 100, 200}, has_fast},
   };
   for (const Dim& d : dims) {
     if (st || !d.used) continue;

@@ -72,6 +72,10 @@ int model_create(const gp_model_desc* d, gp_model** out);
 int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models);
 int bucket_ensure_scratch(Bucket& b);
 BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau);
+// Most lanes one bucket of this model can hold: the kernels address each
+// per-bucket array ([rows][Bp]) with 32-bit byte offsets, so the largest one
+// stays below 2^31 bytes.
+int64_t soa_max_lanes(const gp_model& m);
 
 }  // namespace gp
 

@@ -142,6 +142,8 @@ int find_pool(gp_server* s, const Request& r, int* out) {
     }
   }
   const int cap = s->o.slots_per_pool;
+  if (cap > soa_max_lanes(*m))
+    return fail(GP_ERR_INVALID, "slots_per_pool %d above %lld for this model", cap, (long long)soa_max_lanes(*m));
   const int Bp = (cap + 63) / 64 * 64;
   const size_t bytes = pool_bytes(m, Bp);
   if (s->mem_used + bytes > s->budget) {

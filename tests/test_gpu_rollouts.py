@@ -114,18 +114,19 @@ def test_fault_flags_match_oracle_divergence(gp, oracle):
     assert np.array_equal(bad, flags)
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
 @pytest.mark.parametrize("integrator", [1, 2])
 @pytest.mark.parametrize("name", ["mixed", "cubes"])
-def test_runge_kutta_step_parity(gp, oracle, integrator, name):
-    """RK2 / RK4 (integrators.rs:177-225) run on the plain kernel: one step
-    against the oracle's, FP64 bound."""
+def test_runge_kutta_step_parity(gp, oracle, integrator, name, kernel):
+    """RK2 / RK4 (integrators.rs:177-225) on the plain kernel (1) and on the
+    compiled-topology kernels (2): one step against the oracle's, FP64 bound."""
     from gorilla_physics_amd import batch, workloads
     w = getattr(workloads, name)(n=256)
     if name == "cubes":
         w["q"] = w["q"].copy()
         w["q"][6::7] = 0.46  # corners in contact
-    print(f"case rk{2 * integrator} {name}", flush=True)
-    b, _ = batch.batch_from_workload(w, precision=0)
+    print(f"case rk{2 * integrator} {name} kernel={kernel}", flush=True)
+    b, _ = batch.batch_from_workload(w, precision=0, kernel=kernel)
     q0, v0 = b.get_state()
     b.step(w["dt"], integrator, 1)
     q1, v1 = b.get_state()
@@ -133,6 +134,32 @@ def test_runge_kutta_step_parity(gp, oracle, integrator, name):
     eq, ev, ea = step_errors(oracle, _descs(w), q0, v0, w["tau"], q1, v1, a1, w["dt"], integrator)
     print(f"max e(q') {eq.max():.3g} e(v') {ev.max():.3g} e(a1) {ea.max():.3g}", flush=True)
     assert eq.max() < 1e-10 and ev.max() < 1e-10 and ea.max() < 1e-10
+
+
+@pytest.mark.parametrize("integrator", [1, 2])
+def test_runge_kutta_fast_rollouts(gp, oracle, integrator):
+    """50 RK steps of chains and quadrupeds on the compiled-topology kernels
+    (FP64, 16 steps per launch) against the oracle's rollout; the mixed-precision
+    kernels after one step within the mixed bounds (e(q') <= 1e-5,
+    e(v'), e(a1) <= 2e-3)."""
+    from gorilla_physics_amd import batch, workloads
+    w = workloads.mixed(n=512, seed=8)
+    print(f"case rk{2 * integrator} fast rollout", flush=True)
+    b, _ = batch.batch_from_workload(w, precision=0, kernel=2, steps_per_launch=16)
+    b.step(w["dt"], integrator, 50)
+    qg, vg = b.get_state()
+    rq, rv, fails = oracle.rollout(_descs(w), w["q"], w["v"], w["tau"], w["dt"], 50, integrator=integrator,
+                                   threads=THREADS)
+    assert fails == 0 and not b.faults().any()
+    print(f"50 steps: e(q) {rel_err(qg, rq):.3g} e(v) {rel_err(vg, rv):.3g}", flush=True)
+    assert rel_err(qg, rq) < 1e-9 and rel_err(vg, rv) < 1e-8
+    m, _ = batch.batch_from_workload(w, precision=1, kernel=2)
+    q0, v0 = m.get_state()
+    m.step(w["dt"], integrator, 1)
+    q1, v1 = m.get_state()
+    eq, ev, ea = step_errors(oracle, _descs(w), q0, v0, w["tau"], q1, v1, m.get_vdot(), w["dt"], integrator)
+    print(f"mixed one step: e(q') {eq.max():.3g} e(v') {ev.max():.3g} e(a1) {ea.max():.3g}", flush=True)
+    assert eq.max() <= 1e-5 and ev.max() <= 2e-3 and ea.max() <= 2e-3
 
 
 def test_simulate_loop_matches_oracle(gp, oracle):
