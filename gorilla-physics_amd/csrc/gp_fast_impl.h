@@ -131,27 +131,27 @@ struct Body1 {  // what a 1-DOF body keeps between the sweeps
 };
 
 // Per-body tree parameters of one instance ([NBF][N][Bp] in the bucket).
-// CACHE = false: read from the bucket's array where used (an L1/L2 hit).
-// CACHE = true: loaded into registers once per launch (loads whose values are
-// never used, e.g. spring constants of an all-revolute chain, are dropped).
-// Caching is right while the ~22 values per body fit beside the sweep state:
-// short chains (see cache_params); longer ones re-read every sweep.
-template <class Topo, class R, bool CACHE = false>
+// The first CN bodies' are loaded into registers once per launch (loads whose
+// values are never used, e.g. spring constants of an all-revolute chain, are
+// dropped); the others are read from the bucket's array where used (an L1/L2
+// hit). Caching pays while the ~22 values per body fit beside the sweep
+// state (see cached_bodies).
+template <class Topo, class R, int CN = 0>
 struct FastParams {
   Soa<R> bp;
   int lane;
-  R c[CACHE ? NBF : 1][CACHE ? Topo::N : 1];
+  R c[CN > 0 ? NBF : 1][CN > 0 ? CN : 1];
   __device__ __forceinline__ FastParams(const R* p, int Bp, int lane_) : bp(p, NBF * Topo::N, Bp), lane(lane_) {
-    if constexpr (CACHE) {
+    if constexpr (CN > 0) {
 #pragma unroll
       for (int f = 0; f < NBF; ++f)
 #pragma unroll
-        for (int i = 0; i < Topo::N; ++i) c[f][i] = bp.ld(f * Topo::N + i, lane);
+        for (int i = 0; i < CN; ++i) c[f][i] = bp.ld(f * Topo::N + i, lane);
     }
   }
   __device__ __forceinline__ R operator()(int f, int i) const {
-    if constexpr (CACHE) return c[f][i];
-    else return bp.ld(f * Topo::N + i, lane);
+    if (i < CN) return c[f][i < CN ? i : 0];
+    return bp.ld(f * Topo::N + i, lane);
   }
   __device__ __forceinline__ m3<R> Rt(int i) const {
     m3<R> M;
@@ -167,18 +167,25 @@ struct FastParams {
   }
 };
 
-// Longest chain whose parameters stay in registers for a whole launch without
-// spilling (FP32 / FP64 sweeps, all-revolute; one body fewer with per-lane
-// joint kinds); a lone free body always keeps its one body's.
+// Bodies whose parameters stay in registers for a whole launch without
+// spilling: every body of a chain of up to 9 joints (FP32 sweeps) or 4 (FP64),
+// one fewer with per-lane joint kinds; the first few of a longer chain; a lone
+// free body's one.
 #ifndef GP_PCACHE_F32
 #define GP_PCACHE_F32 9
 #endif
 #ifndef GP_PCACHE_F64
 #define GP_PCACHE_F64 4
 #endif
-template <class Topo, class R, bool ALLREV> constexpr bool cache_params() {
+#ifndef GP_PCACHE_LONG
+#define GP_PCACHE_LONG 6
+#endif
+template <class Topo, class R, bool ALLREV> constexpr int cached_bodies() {
   constexpr int n_max = (sizeof(R) == 4 ? GP_PCACHE_F32 : GP_PCACHE_F64) - (ALLREV ? 0 : 1);
-  return Topo::N == 1 || (Topo::dof(0) == 1 && Topo::N <= n_max);
+  if (Topo::N == 1) return 1;
+  if (Topo::dof(0) != 1) return 0;
+  if (Topo::N <= n_max) return Topo::N;
+  return sizeof(R) == 4 ? (ALLREV ? GP_PCACHE_LONG : GP_PCACHE_LONG / 2) : 0;
 }
 
 // A lone floating body: its articulated inertia is its rigid inertia, the same
@@ -205,6 +212,64 @@ __device__ __forceinline__ bool inverse_inertia(const ai6<R>& I, double (&Iinv)[
   return ok;
 }
 
+// Per-launch constants in registers. TAU: the held torques (topologies whose
+// parameters are all cached). A lone free body: its inverse inertia, its
+// joint's tree transform (FP64), and (with contacts) this lane's contact points
+// (p, k) and up to MAXH half-spaces when the bucket's counts fit (`cached`;
+// else they are read from memory every step).
+template <class Topo, int L, bool CONTACTS, bool TAU>
+struct LaunchConst {
+  static constexpr bool FREE = lone_free_body<Topo>();
+  static constexpr bool TAU_REG = TAU || FREE;
+  static constexpr int MAXC = (FREE && CONTACTS) ? (8 + L - 1) / L : 1;  // points per lane
+  static constexpr int MAXH = (FREE && CONTACTS) ? 2 : 1;
+  double Iinv[FREE ? 21 : 1];
+  double rt[FREE ? 12 : 1];   // R_tree (9), p_tree (3)
+  double tau[TAU_REG ? nv_of<Topo>() : 1];
+  double cp[MAXC][NCF];
+  double hp[MAXH][NHF];
+  int ncl = 0, nh = 0;
+  bool cached = false;
+};
+
+template <class Topo, class R, int L, bool CONTACTS, int PC, bool TAU>
+__device__ __forceinline__ bool load_launch_const(const BucketArgs& a, const FastParams<Topo, R, PC>& P, int lane,
+                                                  int sub, LaunchConst<Topo, L, CONTACTS, TAU>& lc) {
+  using LC = LaunchConst<Topo, L, CONTACTS, TAU>;
+  bool ok = true;
+  if constexpr (LC::TAU_REG) {
+    const Soa<double> tauS(a.tau, nv_of<Topo>(), a.Bp);
+#pragma unroll
+    for (int k = 0; k < nv_of<Topo>(); ++k) lc.tau[k] = tauS.ld(k, lane);
+  }
+  if constexpr (LC::FREE) {
+    ok = inverse_inertia(P.inertia(0), lc.Iinv);
+    const Soa<double> bpS(a.bp, NBF, a.Bp);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) lc.rt[k] = bpS.ld(BF_RT + k, lane);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lc.rt[9 + k] = bpS.ld(BF_PT + k, lane);
+    if constexpr (CONTACTS) {
+      const int c0 = a.cstart[0], c1 = a.cstart[1];
+      lc.cached = c1 - c0 <= LC::MAXC * L && a.nh <= LC::MAXH;  // uniform over the bucket
+      if (lc.cached) {
+        const Soa<double> cpS(a.cp, NCF * a.nc, a.Bp), hpS(a.hp, NHF * a.nh, a.Bp);
+        lc.ncl = c1 - c0 > sub ? (c1 - c0 - sub + L - 1) / L : 0;
+        lc.nh = a.nh;
+#pragma unroll
+        for (int m = 0; m < LC::MAXC; ++m)
+#pragma unroll
+          for (int f = 0; f < NCF; ++f) lc.cp[m][f] = m < lc.ncl ? cpS.ld(f * a.nc + c0 + sub + m * L, lane) : 0.0;
+#pragma unroll
+        for (int h = 0; h < LC::MAXH; ++h)
+#pragma unroll
+          for (int f = 0; f < NHF; ++f) lc.hp[h][f] = h < lc.nh ? hpS.ld(f * a.nh + h, lane) : 0.0;
+      }
+    }
+  }
+  return ok;
+}
+
 // Articulated-body dynamics of one instance with a compile-time topology.
 // SIE = true: one semi-implicit Euler step (integrators.rs:276-319), each
 // joint's v and q advanced as soon as its acceleration is known in the outward
@@ -214,16 +279,15 @@ __device__ __forceinline__ bool inverse_inertia(const ai6<R>& I, double (&Iinv)[
 // RK2 / RK4). Torques are read from memory where used. ALLREV: every 1-DOF
 // joint of the bucket is revolute (no per-lane select). Iinv: the inverse
 // inertia of a lone free body (lone_free_body), else unused.
-template <class Topo, class R, int L, bool CONTACTS, bool ALLREV, bool SIE, bool PC>
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV, bool SIE, int PC, class LC>
 __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<Topo, R, PC>& P, int lane, int sub,
-                                          double* q, double* v, double dt, bool last, const double* Iinv,
-                                          double* qdd_out) {
+                                          double* q, double* v, double dt, bool last, const LC& lc, double* qdd_out) {
   constexpr int N = Topo::N;
   const int Bp = a.Bp;
   const Soa<double> tauS(a.tau, a.nv, Bp), vdotS(a.vdot, a.nv, Bp), bpS(a.bp, NBF * N, Bp);
   const Soa<int> kindS(a.kind, a.nb, Bp);
   const Soa<double> cpS(a.cp, NCF * a.nc, Bp), hpS(a.hp, NHF * a.nh, Bp);
-  auto tau = [&](int k) { return tauS.ld(k, lane); };
+  auto tau = [&](int k) { return LC::TAU_REG ? lc.tau[LC::TAU_REG ? k : 0] : tauS.ld(k, lane); };
   Body1<R> b[N];
   sv<R> vel[N];
   // the floating root (if any) keeps its full transform
@@ -265,9 +329,10 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
       m3<double> Rtd;
       v3<double> ptd;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) Rtd.a[k] = bpS.ld((BF_RT + k) * N + i, lane);
+      for (int k = 0; k < 9; ++k) Rtd.a[k] = lone_free_body<Topo>() ? lc.rt[k] : bpS.ld((BF_RT + k) * N + i, lane);
 #pragma unroll
-      for (int k = 0; k < 3; ++k) (&ptd.x)[k] = bpS.ld((BF_PT + k) * N + i, lane);
+      for (int k = 0; k < 3; ++k)
+        (&ptd.x)[k] = lone_free_body<Topo>() ? lc.rt[9 + k] : bpS.ld((BF_PT + k) * N + i, lane);
       Rm_d = mul(Rtd, Rq);
       p_d = ptd + mul(Rtd, tq);
       const m3<R> Rm = cvt<R>(Rm_d);
@@ -298,23 +363,45 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
       const m3<R> RwR = cvt<R>(Rw[i]);
       sv<R> fe{{R(0), R(0), R(0)}, {R(0), R(0), R(0)}};
       const int c0 = a.cstart[i], c1 = a.cstart[i + 1];
-      for (int cc = c0 + sub; cc < c1; cc += L) {
-        auto cpar = [&](int fld) { return cpS.ld(fld * a.nc + cc, lane); };
-        v3<double> pb{cpar(CF_P), cpar(CF_P + 1), cpar(CF_P + 2)};
-        const R kc = R(cpar(CF_K));
-        v3<double> pw = mul(Rw[i], pb) + tw[i];
-        v3<R> pbR = cvt<R>(pb);
-        v3<R> vw = mul(RwR, V.v + cross(V.w, pbR));
-        for (int h = 0; h < a.nh; ++h) {
-          auto hpar = [&](int fld) { return hpS.ld(fld * a.nh + h, lane); };
-          v3<double> n{hpar(HF_NORMAL), hpar(HF_NORMAL + 1), hpar(HF_NORMAL + 2)};
-          double dist = dot(pw - v3<double>{hpar(HF_POINT), hpar(HF_POINT + 1), hpar(HF_POINT + 2)}, n);
-          if (dist <= kInsideMargin) {
-            v3<R> F = contact_force(R(-dist), cvt<R>(n), vw, kc, R(kHalfspaceK), R(hpar(HF_ALPHA)), R(hpar(HF_MU)));
-            v3<R> Fb = mulT(RwR, F);
-            fe.w = fe.w + cross(pbR, Fb);
-            fe.v = fe.v + Fb;
+      // half-spaces outside, this lane's contact points inside: per half-space
+      // the signed distance of a world point R p + t is (R^T n) . p + n . (t - point),
+      // so each point costs one FP64 dot product; velocity and force only in contact
+      auto touch = [&](const v3<double>& pb, double kc, double dist, const v3<double>& n, R alpha, R mu) {
+        const v3<R> pbR = cvt<R>(pb);
+        const v3<R> vw = mul(RwR, V.v + cross(V.w, pbR));
+        const v3<R> F = contact_force(R(-dist), cvt<R>(n), vw, R(kc), R(kHalfspaceK), alpha, mu);
+        const v3<R> Fb = mulT(RwR, F);
+        fe.w = fe.w + cross(pbR, Fb);
+        fe.v = fe.v + Fb;
+      };
+      if (lc.cached) {
+#pragma unroll
+        for (int h = 0; h < LC::MAXH; ++h) {
+          if (h >= lc.nh) break;
+          const v3<double> n{lc.hp[h][HF_NORMAL], lc.hp[h][HF_NORMAL + 1], lc.hp[h][HF_NORMAL + 2]};
+          const v3<double> nb = mulT(Rw[i], n);
+          const double off = dot(tw[i] - v3<double>{lc.hp[h][HF_POINT], lc.hp[h][HF_POINT + 1], lc.hp[h][HF_POINT + 2]}, n);
+          const R alpha = R(lc.hp[h][HF_ALPHA]), mu = R(lc.hp[h][HF_MU]);
+#pragma unroll
+          for (int m = 0; m < LC::MAXC; ++m) {
+            if (m >= lc.ncl) break;
+            const v3<double> pb{lc.cp[m][CF_P], lc.cp[m][CF_P + 1], lc.cp[m][CF_P + 2]};
+            const double dist = dot(nb, pb) + off;
+            if (dist <= kInsideMargin) touch(pb, lc.cp[m][CF_K], dist, n, alpha, mu);
           }
+        }
+      } else
+      for (int h = 0; h < a.nh && c0 + sub < c1; ++h) {
+        auto hpar = [&](int fld) { return hpS.ld(fld * a.nh + h, lane); };
+        const v3<double> n{hpar(HF_NORMAL), hpar(HF_NORMAL + 1), hpar(HF_NORMAL + 2)};
+        const v3<double> nb = mulT(Rw[i], n);
+        const double off = dot(tw[i] - v3<double>{hpar(HF_POINT), hpar(HF_POINT + 1), hpar(HF_POINT + 2)}, n);
+        const R alpha = R(hpar(HF_ALPHA)), mu = R(hpar(HF_MU));
+        for (int cc = c0 + sub; cc < c1; cc += L) {
+          auto cpar = [&](int fld) { return cpS.ld(fld * a.nc + cc, lane); };
+          const v3<double> pb{cpar(CF_P), cpar(CF_P + 1), cpar(CF_P + 2)};
+          const double dist = dot(nb, pb) + off;
+          if (dist <= kInsideMargin) touch(pb, cpar(CF_K), dist, n, alpha, mu);
         }
       }
       if (L > 1) {
@@ -414,7 +501,7 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
         for (int rr = 0; rr < 6; ++rr) {
           double s = 0.0;
 #pragma unroll
-          for (int cc = 0; cc < 6; ++cc) s += Iinv[sym6(rr, cc)] * u6[cc];
+          for (int cc = 0; cc < 6; ++cc) s += lc.Iinv[sym6(rr, cc)] * u6[cc];
           x[rr] = s - apd[rr];
         }
       } else {
@@ -470,12 +557,12 @@ __device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, doub
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = vS.ld(k, inst);
   bool ok = true;
-  constexpr bool PC = cache_params<Topo, R, ALLREV>();
+  constexpr int PC = cached_bodies<Topo, R, ALLREV>();
   const FastParams<Topo, R, PC> P(pick<R>(a.bp, a.bpf), Bp, inst);
-  double Iinv[lone_free_body<Topo>() ? 21 : 1];
-  if constexpr (lone_free_body<Topo>()) ok = inverse_inertia(P.inertia(0), Iinv);
+  LaunchConst<Topo, L, CONTACTS, PC == Topo::N> lc;
+  ok = load_launch_const(a, P, inst, sub, lc);
   for (int s = 0; s < my_steps; ++s)
-    ok = fast_step<Topo, R, L, CONTACTS, ALLREV, true, PC>(a, P, inst, sub, q, v, dt, s + 1 == my_steps, Iinv,
+    ok = fast_step<Topo, R, L, CONTACTS, ALLREV, true, PC>(a, P, inst, sub, q, v, dt, s + 1 == my_steps, lc,
                                                           nullptr) && ok;
   if (sub != 0) return;
 #pragma unroll
@@ -538,10 +625,10 @@ __device__ __forceinline__ void fast_rk_body(const BucketArgs& a, int tid, doubl
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = vS.ld(k, inst);
   bool ok = true;
-  constexpr bool PC = cache_params<Topo, R, ALLREV>();
+  constexpr int PC = cached_bodies<Topo, R, ALLREV>();
   const FastParams<Topo, R, PC> P(pick<R>(a.bp, a.bpf), Bp, inst);
-  double Iinv[lone_free_body<Topo>() ? 21 : 1];
-  if constexpr (lone_free_body<Topo>()) ok = inverse_inertia(P.inertia(0), Iinv);
+  LaunchConst<Topo, L, CONTACTS, PC == Topo::N> lc;
+  ok = load_launch_const(a, P, inst, sub, lc);
   const bool rk4 = integrator == GP_RUNGE_KUTTA_4;
   const int stages = rk4 ? 4 : 2;
   double qs[NQ], vs[NV], ak[NV], sum[NV];
@@ -552,7 +639,7 @@ __device__ __forceinline__ void fast_rk_body(const BucketArgs& a, int tid, doubl
     for (int k = 0; k < NV; ++k) vs[k] = v[k];
 #pragma unroll 1
     for (int st = 0; st < stages; ++st) {
-      ok = fast_step<Topo, R, L, CONTACTS, ALLREV, false, PC>(a, P, inst, sub, qs, vs, 0.0, false, Iinv, ak) && ok;
+      ok = fast_step<Topo, R, L, CONTACTS, ALLREV, false, PC>(a, P, inst, sub, qs, vs, 0.0, false, lc, ak) && ok;
       if (st == 0) {
         if (s + 1 == my_steps && sub == 0) {
 #pragma unroll
@@ -852,7 +939,7 @@ __global__ void __launch_bounds__(256) grouped_sie_kernel(const GroupTable* __re
   while (b + 1 < t->n && (int)blockIdx.x >= t->block_start[b + 1]) ++b;  // uniform per workgroup
   const BucketArgs a = t->args[b];
   const int tid = ((int)blockIdx.x - t->block_start[b]) * (int)blockDim.x + (int)threadIdx.x;
-  grouped_dispatch<R>(a, t->topo[b], tid, a.dt > 0.0 ? a.dt : dt, n_steps);
+  grouped_dispatch<R>(a, t->topo[b], tid, a.dt > 0.0 ? a.dt : dt, a.n_steps > 0 ? a.n_steps : n_steps);
 }
 
 

@@ -62,6 +62,7 @@ struct BucketArgs {
   int nb, nq, nv, nc, nh;
   int lanes;    // lanes per instance (FAST kernels)
   int all_rev;  // every 1-DOF joint of every lane is revolute
+  int n_steps;  // steps of this bucket in a grouped launch (serving pools); 0 = the launch's
   double g;
   double dt;    // step of this bucket (serving pools); 0 = the launch's dt
   // topology (device, uniform over the bucket)

@@ -7,9 +7,13 @@
 //      memory budget allows it (memory-based admission); all admissions of a
 //      round go to the device in one pinned host->device copy and one admit
 //      launch per pool;
-//   3. one grouped launch advances every busy slot of every pool by up to
-//      chunk_steps steps (a slot runs what is left of its horizon; the kernel
-//      counts its remaining steps down, idle slots exit at once);
+//   3. one grouped launch advances every busy slot of every pool by up to its
+//      pool's steps for the round (a slot runs what is left of its horizon;
+//      the kernel counts its remaining steps down, idle slots exit at once).
+//      The slowest busy pool runs chunk_steps; a pool whose step costs less
+//      runs proportionally more (up to 32x), so every pool's part of the
+//      launch takes about as long and a request's latency follows its own
+//      mechanism's cost rather than the costliest one in the mix;
 //   4. slots whose horizon ended are packed by one gather launch per pool and
 //      copied back in one device->host copy; their requests complete with
 //      latency = completion - arrival on the server clock.
@@ -54,12 +58,32 @@ struct Pool {
   uint64_t sig = 0;
   double dt = 0;
   bool all_rev = false;
+  double cost = 1;  // relative time of one step of one wave (step_cost)
+  int chunk = 0;    // steps per round
   Bucket b;
   std::vector<int64_t> slot_req;  // request id per slot, -1 = free
   std::vector<int> free_slots;
   int busy = 0;
   size_t bytes = 0;
 };
+
+constexpr int kMaxChunkScale = 32;
+
+// Time of one step of one wave of a pool's kernel in microseconds, by
+// compiled topology and precision (measured on MI355X with 16,384 instances
+// per topology; entries for 5, 7, 9 and 11 joints interpolated). Per-lane
+// revolute/prismatic kinds add about a fifth. Only the ratios matter.
+double step_cost(const Bucket& b, int precision, int lanes_per_tree) {
+  static const double chain_mixed[12] = {0.25, 1.07, 2.13, 3.08, 4.2, 5.27, 6.4, 7.6, 11.5, 15.5, 17.9, 20.2};
+  static const double chain_f64[12] = {0.5, 1.35, 2.77, 4.16, 7.3, 10.5, 14.2, 17.9, 33.4, 48.9, 61.7, 74.5};
+  const bool mixed = precision == GP_PREC_MIXED;
+  const int t = b.fast_topo;
+  if (t >= 1 && t <= 12) return (mixed ? chain_mixed : chain_f64)[t - 1] * (b.all_rev ? 1.0 : 1.2);
+  if (t == TOPO_QUAD && lanes_per_tree == 1) return mixed ? 29.4 : 116.0;
+  if (t == TOPO_QUAD) return mixed ? 7.0 : 14.3;   // one lane per leg
+  if (t == TOPO_FREE) return mixed ? 1.4 : 1.7;    // four lanes per body, 8 contact points
+  return mixed ? 20.0 : 75.0;
+}
 
 template <class T>
 size_t round_up(size_t x) {
@@ -170,6 +194,7 @@ int find_pool(gp_server* s, const Request& r, int* out) {
   std::vector<const gp_model*> lanes(cap, m);
   int st = bucket_build(p->b, lanes);
   if (st) return st;
+  p->cost = step_cost(p->b, s->o.precision, s->o.lanes_per_tree);
   hipError_t e = hipMalloc((void**)&p->b.d_active, sizeof(int) * Bp);
   if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(active): %s", hipGetErrorString(e));
   e = hipMemset(p->b.d_active, 0, sizeof(int) * Bp);
@@ -195,6 +220,7 @@ int rebuild_group(gp_server* s) {
     const int lanes = fast_lanes(p.b.fast_topo, s->o.lanes_per_mech, s->o.lanes_per_tree);
     BucketArgs a = bucket_args(p.b, lanes, nullptr);
     a.dt = p.dt;
+    a.n_steps = p.chunk;
     t.args[t.n] = a;
     t.topo[t.n] = p.b.fast_topo;
     t.block_start[t.n] = blocks;
@@ -372,6 +398,14 @@ int run_rounds(gp_server* s, int max_rounds, int* rounds_done) {
       if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
       continue;
     }
+    // steps per pool this round: the costliest busy pool runs C
+    double cmax = 0;
+    for (auto& pp : s->pools)
+      if (pp->busy) cmax = std::max(cmax, pp->cost);
+    for (auto& pp : s->pools) {
+      const int ch = pp->busy ? std::min(C * kMaxChunkScale, std::max(C, (int)std::lround(C * cmax / pp->cost))) : C;
+      if (ch != pp->chunk) { pp->chunk = ch; s->group_dirty = true; }
+    }
     if (s->group_dirty) {
       int st = rebuild_group(s);
       if (st) return st;
@@ -388,7 +422,7 @@ int run_rounds(gp_server* s, int max_rounds, int* rounds_done) {
         const int64_t id = p.slot_req[l];
         if (id < 0) continue;
         Request& r = s->req[id];
-        const int64_t k = std::min<int64_t>(C, r.remaining);
+        const int64_t k = std::min<int64_t>(p.chunk, r.remaining);
         r.remaining -= k;
         s->env_steps += (double)k;
         if (r.remaining == 0) fin[(int)pi].push_back(id);
