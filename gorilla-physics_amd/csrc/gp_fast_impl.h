@@ -126,7 +126,7 @@ struct Body1 {  // what a 1-DOF body keeps between the sweeps
   R cv0, cv1;      // linear x, y (z is 0)
   sv<R> pA;        // articulated bias force
   sv<R> U;         // I^A S
-  double D, u;     // S^T I^A S and tau - S^T p^A
+  double Dinv, u;  // 1 / (S^T I^A S) and tau - S^T p^A
   int rev;  // 1 revolute, 0 prismatic
 };
 
@@ -427,7 +427,7 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
     if (Topo::dof(i) == 1) {
       Body1<R>& B = b[i];
       B.U = B.rev ? col(IA, 2) : col(IA, 5);
-      B.D = double(sel(B.rev, IA.A.zz, IA.C.zz));
+      B.Dinv = 1.0 / double(sel(B.rev, IA.A.zz, IA.C.zz));
       double tq = tau(voff_of<Topo>(i));
       if (!ALLREV) {
         // prismatic spring, branch-free: a revolute joint or k = 0 adds -0
@@ -435,9 +435,9 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
         tq += -sk * (q[qoff_of<Topo>(i)] - double(P(BF_SL, i)));
       }
       B.u = tq - double(sel(B.rev, B.pA.w.z, B.pA.v.z));
-      ok = ok && (B.D > 0.0);
+      ok = ok && (B.Dinv > 0.0);
       if (pr >= 0) {
-        const R inv = R(1.0 / B.D);
+        const R inv = R(B.Dinv);
         const sv<R> uD{inv * B.U.w, inv * B.U.v};
         const ai6<R> Ia = rank1_down(IA, B.U, uD);
         const R uR = R(B.u);
@@ -477,7 +477,7 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
       joint_frame(P.Rt(i), P.pt(i), B.c, B.s, B.d, E, r);
       sv<R> ap = xmotion(E, r, pr < 0 ? a0 : accel[pp]);
       ap.w.x += B.cw0; ap.w.y += B.cw1; ap.v.x += B.cv0; ap.v.y += B.cv1;
-      const double qdd = (B.u - double(dot6(B.U, ap))) / B.D;
+      const double qdd = (B.u - double(dot6(B.U, ap))) * B.Dinv;
       const int vo = voff_of<Topo>(i);
       ok = ok && isfinite(qdd);
       if constexpr (SIE) {
@@ -792,7 +792,7 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
       ai6<R> IA = P.inertia(i);
       if (k < DEPTH - 1) add_to(IA, acc);
       B.U = B.rev ? col(IA, 2) : col(IA, 5);
-      B.D = double(sel(B.rev, IA.A.zz, IA.C.zz));
+      B.Dinv = 1.0 / double(sel(B.rev, IA.A.zz, IA.C.zz));
       double tq = tl[k];
       if (!ALLREV) {
         // prismatic spring, branch-free: a revolute joint or k = 0 adds -0
@@ -800,8 +800,8 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
         tq += -sk * (ql[k] - double(P(BF_SL, i)));
       }
       B.u = tq - double(sel(B.rev, B.pA.w.z, B.pA.v.z));
-      ok = ok && (B.D > 0.0);
-      const R inv = R(1.0 / B.D);
+      ok = ok && (B.Dinv > 0.0);
+      const R inv = R(B.Dinv);
       const sv<R> uD{inv * B.U.w, inv * B.U.v};
       const ai6<R> Ia = rank1_down(IA, B.U, uD);
       const R uR = R(B.u);
@@ -853,7 +853,7 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
       joint_frame(P.Rt(i), P.pt(i), B.c, B.s, B.d, E, r);
       sv<R> ap = xmotion(E, r, accel);
       ap.w.x += B.cw0; ap.w.y += B.cw1; ap.v.x += B.cv0; ap.v.y += B.cv1;
-      const double qdd = (B.u - double(dot6(B.U, ap))) / B.D;
+      const double qdd = (B.u - double(dot6(B.U, ap))) * B.Dinv;
       al[k] = qdd;
       ap.w.z = B.rev ? ap.w.z + R(qdd) : ap.w.z;
       ap.v.z = B.rev ? ap.v.z : ap.v.z + R(qdd);

@@ -239,32 +239,20 @@ int model_create(const gp_model_desc* d, gp_model** out) {
 Bucket::~Bucket() { release(); }
 
 void Bucket::release() {
-  void* ptrs[] = {d_parent, d_qoff, d_voff, d_cstart, d_kind, d_bp, d_bpf, d_cp, d_cpf, d_hp, d_hpf,
-                  d_q, d_v, d_tau, d_tau_tmp, d_vdot, d_fault, d_scratch, d_active};
+  void* ptrs[] = {d_arena, d_scratch, d_active};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  d_arena = nullptr;
   d_parent = d_qoff = d_voff = d_cstart = d_kind = d_fault = d_active = nullptr;
   d_bp = d_cp = d_hp = d_q = d_v = d_tau = d_tau_tmp = d_vdot = d_scratch = nullptr;
   d_bpf = d_cpf = d_hpf = nullptr;
 }
 
-template <class T>
-static int dalloc(T** p, size_t n) {
-  if (n == 0) n = 1;
-  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
-  if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
-  return GP_OK;
-}
-template <class T>
-static int upload(T* d, const std::vector<T>& h) {
-  if (h.empty()) return GP_OK;
-  hipError_t e = hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
-  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "hipMemcpy H2D: %s", hipGetErrorString(e));
-  return GP_OK;
-}
-
-// Allocate and fill one bucket from the models of its lanes.
-int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models) {
+// Allocate and fill one bucket from the models of its lanes: every array of
+// the bucket lives in one device allocation (256-byte aligned pieces), filled
+// by one host-to-device copy. fill_lanes = false (serving pools, whose slots
+// are written at admission) skips the per-lane packing: the lanes start zeroed.
+int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, bool fill_lanes) {
   const gp_model* m0 = lane_models[0];
   b.nb = m0->n; b.nq = m0->nq; b.nv = m0->nv; b.nc = m0->nc; b.nh = m0->nh; b.g = m0->g;
   b.parent = m0->parent; b.qoff = m0->qoff; b.voff = m0->voff; b.dofcls = m0->dofcls; b.cstart = m0->cstart;
@@ -273,51 +261,85 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models) {
   b.B = (int)lane_models.size();
   b.Bp = (b.B + 63) / 64 * 64;
   const size_t Bp = b.Bp;
-  std::vector<int> kind((size_t)b.nb * Bp, K_FIXED);
-  std::vector<double> bp((size_t)NBF * b.nb * Bp, 0.0), cp((size_t)NCF * b.nc * Bp, 0.0), hp((size_t)NHF * b.nh * Bp, 0.0);
+  const size_t nq1 = std::max(1, b.nq), nv1 = std::max(1, b.nv);
+  // layout
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = at;
+    at += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_parent = take(4 * b.nb), o_qoff = take(4 * b.nb), o_voff = take(4 * b.nb),
+               o_cstart = take(4 * (b.nb + 1)), o_kind = take(4 * b.nb * Bp), o_bp = take(8 * NBF * b.nb * Bp),
+               o_bpf = take(4 * NBF * b.nb * Bp), o_cp = take(8 * NCF * b.nc * Bp), o_cpf = take(4 * NCF * b.nc * Bp),
+               o_hp = take(8 * NHF * b.nh * Bp), o_hpf = take(4 * NHF * b.nh * Bp), o_q = take(8 * nq1 * Bp),
+               o_v = take(8 * nv1 * Bp), o_tau = take(8 * nv1 * Bp), o_tau_tmp = take(8 * nv1 * Bp),
+               o_vdot = take(8 * nv1 * Bp), o_fault = take(4 * Bp);
+  std::vector<char> h(at, 0);
+  auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(h.data() + off, src, bytes); };
+  put(o_parent, b.parent.data(), 4 * b.nb);
+  put(o_qoff, b.qoff.data(), 4 * b.nb);
+  put(o_voff, b.voff.data(), 4 * b.nb);
+  put(o_cstart, b.cstart.data(), 4 * (b.nb + 1));
+  int* kind = (int*)(h.data() + o_kind);
+  double* bp = (double*)(h.data() + o_bp);
+  float* bpf = (float*)(h.data() + o_bpf);
+  double* cp = (double*)(h.data() + o_cp);
+  float* cpf = (float*)(h.data() + o_cpf);
+  double* hp = (double*)(h.data() + o_hp);
+  float* hpf = (float*)(h.data() + o_hpf);
   for (int l = 0; l < b.B; ++l) {
     const gp_model* m = lane_models[l];
+    for (int i = 0; i < b.nb; ++i)
+      if (m->kind[i] == K_PRI) b.all_rev = false;
+    if (!fill_lanes) continue;
     for (int i = 0; i < b.nb; ++i) {
       kind[i * Bp + l] = m->kind[i];
-      if (m->kind[i] == K_PRI) b.all_rev = false;
-      for (int f = 0; f < NBF; ++f) bp[((size_t)f * b.nb + i) * Bp + l] = m->bp[(size_t)i * NBF + f];
+      for (int f = 0; f < NBF; ++f) {
+        const double x = m->bp[(size_t)i * NBF + f];
+        bp[((size_t)f * b.nb + i) * Bp + l] = x;
+        bpf[((size_t)f * b.nb + i) * Bp + l] = (float)x;
+      }
     }
     for (int c = 0; c < b.nc; ++c)
-      for (int f = 0; f < NCF; ++f) cp[((size_t)f * b.nc + c) * Bp + l] = m->cp[(size_t)c * NCF + f];
-    for (int h = 0; h < b.nh; ++h)
-      for (int f = 0; f < NHF; ++f) hp[((size_t)f * b.nh + h) * Bp + l] = m->hp[(size_t)h * NHF + f];
+      for (int f = 0; f < NCF; ++f) {
+        const double x = m->cp[(size_t)c * NCF + f];
+        cp[((size_t)f * b.nc + c) * Bp + l] = x;
+        cpf[((size_t)f * b.nc + c) * Bp + l] = (float)x;
+      }
+    for (int hh = 0; hh < b.nh; ++hh)
+      for (int f = 0; f < NHF; ++f) {
+        const double x = m->hp[(size_t)hh * NHF + f];
+        hp[((size_t)f * b.nh + hh) * Bp + l] = x;
+        hpf[((size_t)f * b.nh + hh) * Bp + l] = (float)x;
+      }
   }
-  std::vector<float> bpf(bp.begin(), bp.end()), cpf(cp.begin(), cp.end()), hpf(hp.begin(), hp.end());
-  int st;
-  if ((st = dalloc(&b.d_parent, b.nb)) || (st = dalloc(&b.d_qoff, b.nb)) || (st = dalloc(&b.d_voff, b.nb)) ||
-      (st = dalloc(&b.d_cstart, b.nb + 1)) || (st = dalloc(&b.d_kind, kind.size())) || (st = dalloc(&b.d_bp, bp.size())) ||
-      (st = dalloc(&b.d_bpf, bpf.size())) || (st = dalloc(&b.d_cp, cp.size())) || (st = dalloc(&b.d_cpf, cpf.size())) ||
-      (st = dalloc(&b.d_hp, hp.size())) || (st = dalloc(&b.d_hpf, hpf.size())) ||
-      (st = dalloc(&b.d_q, (size_t)std::max(1, b.nq) * Bp)) || (st = dalloc(&b.d_v, (size_t)std::max(1, b.nv) * Bp)) ||
-      (st = dalloc(&b.d_tau, (size_t)std::max(1, b.nv) * Bp)) || (st = dalloc(&b.d_tau_tmp, (size_t)std::max(1, b.nv) * Bp)) ||
-      (st = dalloc(&b.d_vdot, (size_t)std::max(1, b.nv) * Bp)) || (st = dalloc(&b.d_fault, Bp)))
-    return st;
-  if ((st = upload(b.d_parent, b.parent)) || (st = upload(b.d_qoff, b.qoff)) || (st = upload(b.d_voff, b.voff)) ||
-      (st = upload(b.d_cstart, b.cstart)) || (st = upload(b.d_kind, kind)) || (st = upload(b.d_bp, bp)) ||
-      (st = upload(b.d_bpf, bpf)) || (st = upload(b.d_cp, cp)) || (st = upload(b.d_cpf, cpf)) ||
-      (st = upload(b.d_hp, hp)) || (st = upload(b.d_hpf, hpf)))
-    return st;
   // zero state (mechanism.rs:68-88): q = 0, floating q = identity pose, v = 0, tau = 0
-  std::vector<double> q0((size_t)std::max(1, b.nq) * Bp, 0.0);
+  double* q0 = (double*)(h.data() + o_q);
   for (int i = 0; i < b.nb; ++i)
     if (b.dofcls[i] == 6)
       for (size_t l = 0; l < Bp; ++l) q0[(size_t)(b.qoff[i] + 3) * Bp + l] = 1.0;
-  if ((st = upload(b.d_q, q0))) return st;
-  HIP_TRY(hipMemset(b.d_v, 0, sizeof(double) * std::max(1, b.nv) * Bp));
-  HIP_TRY(hipMemset(b.d_tau, 0, sizeof(double) * std::max(1, b.nv) * Bp));
-  HIP_TRY(hipMemset(b.d_vdot, 0, sizeof(double) * std::max(1, b.nv) * Bp));
-  HIP_TRY(hipMemset(b.d_fault, 0, sizeof(int) * Bp));
+  hipError_t e = hipMalloc(&b.d_arena, at);
+  if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(%zu bytes): %s", at, hipGetErrorString(e));
+  e = hipMemcpy(b.d_arena, h.data(), at, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "hipMemcpy H2D: %s", hipGetErrorString(e));
+  char* d = (char*)b.d_arena;
+  b.d_parent = (int*)(d + o_parent); b.d_qoff = (int*)(d + o_qoff); b.d_voff = (int*)(d + o_voff);
+  b.d_cstart = (int*)(d + o_cstart); b.d_kind = (int*)(d + o_kind);
+  b.d_bp = (double*)(d + o_bp); b.d_bpf = (float*)(d + o_bpf); b.d_cp = (double*)(d + o_cp);
+  b.d_cpf = (float*)(d + o_cpf); b.d_hp = (double*)(d + o_hp); b.d_hpf = (float*)(d + o_hpf);
+  b.d_q = (double*)(d + o_q); b.d_v = (double*)(d + o_v); b.d_tau = (double*)(d + o_tau);
+  b.d_tau_tmp = (double*)(d + o_tau_tmp); b.d_vdot = (double*)(d + o_vdot); b.d_fault = (int*)(d + o_fault);
+  b.arena_bytes = at;
   return GP_OK;
 }
 
 int bucket_ensure_scratch(Bucket& b) {
   if (b.d_scratch) return GP_OK;
-  return dalloc(&b.d_scratch, plain_scratch_doubles(b.nb, b.nq, b.nv, b.Bp));
+  const size_t bytes = sizeof(double) * std::max<size_t>(1, plain_scratch_doubles(b.nb, b.nq, b.nv, b.Bp));
+  hipError_t e = hipMalloc((void**)&b.d_scratch, bytes);
+  if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  return GP_OK;
 }
 
 int64_t soa_max_lanes(const gp_model& m) {

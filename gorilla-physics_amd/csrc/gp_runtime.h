@@ -42,6 +42,8 @@ struct Bucket {
   int* d_fault = nullptr;
   int* d_active = nullptr;
   double* d_scratch = nullptr;
+  void* d_arena = nullptr;  // one allocation holding every array above except scratch and active
+  size_t arena_bytes = 0;
   Bucket() = default;
   Bucket(const Bucket&) = delete;
   Bucket& operator=(const Bucket&) = delete;
@@ -56,7 +58,8 @@ struct Bucket {
       d_parent = o.d_parent; d_qoff = o.d_qoff; d_voff = o.d_voff; d_cstart = o.d_cstart; d_kind = o.d_kind;
       d_bp = o.d_bp; d_cp = o.d_cp; d_hp = o.d_hp; d_bpf = o.d_bpf; d_cpf = o.d_cpf; d_hpf = o.d_hpf;
       d_q = o.d_q; d_v = o.d_v; d_tau = o.d_tau; d_tau_tmp = o.d_tau_tmp; d_vdot = o.d_vdot; d_fault = o.d_fault;
-      d_active = o.d_active; d_scratch = o.d_scratch;
+      d_active = o.d_active; d_scratch = o.d_scratch; d_arena = o.d_arena; arena_bytes = o.arena_bytes;
+      o.d_arena = nullptr;
       o.d_parent = o.d_qoff = o.d_voff = o.d_cstart = o.d_kind = o.d_fault = o.d_active = nullptr;
       o.d_bp = o.d_cp = o.d_hp = o.d_q = o.d_v = o.d_tau = o.d_tau_tmp = o.d_vdot = o.d_scratch = nullptr;
       o.d_bpf = o.d_cpf = o.d_hpf = nullptr;
@@ -69,7 +72,7 @@ struct Bucket {
 
 int fail(int code, const char* fmt, ...);
 int model_create(const gp_model_desc* d, gp_model** out);
-int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models);
+int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, bool fill_lanes = true);
 int bucket_ensure_scratch(Bucket& b);
 BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau);
 // Most lanes one bucket of this model can hold: the kernels address each

@@ -192,7 +192,7 @@ int find_pool(gp_server* s, const Request& r, int* out) {
   p->all_rev = m->all_rev;
   p->dt = r.dt;
   std::vector<const gp_model*> lanes(cap, m);
-  int st = bucket_build(p->b, lanes);
+  int st = bucket_build(p->b, lanes, false);
   if (st) return st;
   p->cost = step_cost(p->b, s->o.precision, s->o.lanes_per_tree);
   hipError_t e = hipMalloc((void**)&p->b.d_active, sizeof(int) * Bp);
