@@ -67,7 +67,8 @@ def run_variant(w, models, steps, warmup, autotune=False, **opts):
     b.set_tau(w["tau"])
     tuned_ms = None
     if autotune:
-        _, tuned_ms = b.autotune(w["dt"], 0, trial_steps=200)
+        # trials long enough that launch-to-launch noise does not pick the options
+        _, tuned_ms = b.autotune(w["dt"], 0, trial_steps=min(steps, 1000))
     o = b.get_opts()
     chosen = dict(block_size=o.block_size, lanes_per_mech=o.lanes_per_mech, lanes_per_tree=o.lanes_per_tree,
                   steps_per_launch=o.steps_per_launch, use_graph=o.use_graph)

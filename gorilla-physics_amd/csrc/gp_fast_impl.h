@@ -88,6 +88,25 @@ struct Soa {
     else
       return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, lane * 4, row * pitch, 0));
   }
+  // row + xrow with xrow varying across lanes (a per-lane part must go into the
+  // VGPR offset: a divergent scalar offset would need a loop over its values)
+  __device__ __forceinline__ T ldx(int row, int lane, int xrow) const {
+    const int voff = lane * (int)sizeof(T) + xrow * pitch;
+    if constexpr (sizeof(T) == 8)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, row * pitch, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, row * pitch, 0));
+  }
+  __device__ __forceinline__ void stx(int row, int lane, int xrow, T x) const {
+    const int voff = lane * (int)sizeof(T) + xrow * pitch;
+    if constexpr (sizeof(T) == 8) {
+      using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V, x), r, voff, row * pitch, 0);
+    } else {
+      using V = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0));
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(V, x), r, voff, row * pitch, 0);
+    }
+  }
   __device__ __forceinline__ void st(int row, int lane, T x) const {
     if constexpr (sizeof(T) == 8) {
       using V = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
@@ -140,18 +159,20 @@ template <class Topo, class R, int CN = 0>
 struct FastParams {
   Soa<R> bp;
   int lane;
+  int xbody;  // per-lane shift of the body index (a lane that runs one leg of a tree)
   R c[CN > 0 ? NBF : 1][CN > 0 ? CN : 1];
-  __device__ __forceinline__ FastParams(const R* p, int Bp, int lane_) : bp(p, NBF * Topo::N, Bp), lane(lane_) {
+  __device__ __forceinline__ FastParams(const R* p, int Bp, int lane_, int xbody_ = 0)
+      : bp(p, NBF * Topo::N, Bp), lane(lane_), xbody(xbody_) {
     if constexpr (CN > 0) {
 #pragma unroll
       for (int f = 0; f < NBF; ++f)
 #pragma unroll
-        for (int i = 0; i < CN; ++i) c[f][i] = bp.ld(f * Topo::N + i, lane);
+        for (int i = 0; i < CN; ++i) c[f][i] = bp.ldx(f * Topo::N + i, lane, xbody);
     }
   }
   __device__ __forceinline__ R operator()(int f, int i) const {
     if (i < CN) return c[f][i < CN ? i : 0];
-    return bp.ld(f * Topo::N + i, lane);
+    return bp.ldx(f * Topo::N + i, lane, xbody);
   }
   __device__ __forceinline__ m3<R> Rt(int i) const {
     m3<R> M;
@@ -721,7 +742,10 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
     my_steps = min(my_steps, left);
     if (leg == 0) a.active[inst] = left - my_steps;
   }
+  // this lane's leg: bodies 1 + legrow + k, q rows 7 + legrow + k, v rows 6 + legrow + k
+  const int legrow = leg * DEPTH;
   const FastParams<TopoTree<LEGS, DEPTH>, R> P(pick<R>(a.bp, a.bpf), Bp, inst);
+  const FastParams<TopoTree<LEGS, DEPTH>, R> PL(pick<R>(a.bp, a.bpf), Bp, inst, legrow);
   const Soa<double> qS(a.q, 7 + LEGS * DEPTH, Bp), vS(a.v, 6 + LEGS * DEPTH, Bp), tauS(a.tau, 6 + LEGS * DEPTH, Bp),
       vdotS(a.vdot, 6 + LEGS * DEPTH, Bp), bpS(a.bp, NBF * N, Bp);
   const Soa<int> kindS(a.kind, N, Bp);
@@ -738,11 +762,10 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
   }
 #pragma unroll
   for (int k = 0; k < DEPTH; ++k) {
-    const int j = leg * DEPTH + k;
-    ql[k] = qS.ld((7 + j), inst);
-    vl[k] = vS.ld((6 + j), inst);
-    tl[k] = tauS.ld((6 + j), inst);
-    al[k] = vdotS.ld((6 + j), inst);
+    ql[k] = qS.ldx(7 + k, inst, legrow);
+    vl[k] = vS.ldx(6 + k, inst, legrow);
+    tl[k] = tauS.ldx(6 + k, inst, legrow);
+    al[k] = vdotS.ldx(6 + k, inst, legrow);
   }
   bool ok = true;
   for (int s = 0; s < my_steps; ++s) {
@@ -764,18 +787,18 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
     sv<R> vel = V0;
 #pragma unroll
     for (int k = 0; k < DEPTH; ++k) {
-      const int i = 1 + leg * DEPTH + k;
-      const bool rev = ALLREV || kindS.ld(i, inst) == K_REV;
+      const int i = 1 + k;  // in PL, shifted to this leg
+      const bool rev = ALLREV || kindS.ldx(i, inst, legrow) == K_REV;
       b[k].rev = rev;
       sincos_q(rev ? ql[k] : 0.0, &b[k].s, &b[k].c);
       b[k].d = rev ? R(0) : R(ql[k]);
       m3<R> E;
       v3<R> r;
-      joint_frame(P.Rt(i), P.pt(i), b[k].c, b[k].s, b[k].d, E, r);
+      joint_frame(PL.Rt(i), PL.pt(i), b[k].c, b[k].s, b[k].d, E, r);
       const R qd = R(vl[k]);
       sv<R> vj{{R(0), R(0), rev ? qd : R(0)}, {R(0), R(0), rev ? R(0) : qd}};
       vel = xmotion(E, r, vel) + vj;
-      b[k].pA = crf(vel, mul(P.inertia(i), vel));
+      b[k].pA = crf(vel, mul(PL.inertia(i), vel));
       b[k].cw0 = rev ? qd * vel.w.y : R(0);
       b[k].cw1 = rev ? -qd * vel.w.x : R(0);
       b[k].cv0 = qd * sel(rev, vel.v.y, vel.w.y);
@@ -787,17 +810,17 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
     sv<R> cpA;
 #pragma unroll
     for (int k = DEPTH - 1; k >= 0; --k) {
-      const int i = 1 + leg * DEPTH + k;
+      const int i = 1 + k;  // in PL, shifted to this leg
       Body1<R>& B = b[k];
-      ai6<R> IA = P.inertia(i);
+      ai6<R> IA = PL.inertia(i);
       if (k < DEPTH - 1) add_to(IA, acc);
       B.U = B.rev ? col(IA, 2) : col(IA, 5);
       B.Dinv = 1.0 / double(sel(B.rev, IA.A.zz, IA.C.zz));
       double tq = tl[k];
       if (!ALLREV) {
         // prismatic spring, branch-free: a revolute joint or k = 0 adds -0
-        const double sk = B.rev ? 0.0 : double(P(BF_SK, i));
-        tq += -sk * (ql[k] - double(P(BF_SL, i)));
+        const double sk = B.rev ? 0.0 : double(PL(BF_SK, i));
+        tq += -sk * (ql[k] - double(PL(BF_SL, i)));
       }
       B.u = tq - double(sel(B.rev, B.pA.w.z, B.pA.v.z));
       ok = ok && (B.Dinv > 0.0);
@@ -809,7 +832,7 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
       const sv<R> pa = B.pA + mul(Ia, cbv) + sv<R>{uR * uD.w, uR * uD.v};
       m3<R> E;
       v3<R> r;
-      joint_frame(P.Rt(i), P.pt(i), B.c, B.s, B.d, E, r);
+      joint_frame(PL.Rt(i), PL.pt(i), B.c, B.s, B.d, E, r);
       if (k > 0) {
         acc = xtix(E, r, Ia);
         b[k - 1].pA = b[k - 1].pA + xtforce(E, r, pa);
@@ -846,11 +869,11 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
     sv<R> accel = ap0 + sv<R>{{R(x[0]), R(x[1]), R(x[2])}, {R(x[3]), R(x[4]), R(x[5])}};
 #pragma unroll
     for (int k = 0; k < DEPTH; ++k) {
-      const int i = 1 + leg * DEPTH + k;
+      const int i = 1 + k;  // in PL, shifted to this leg
       const Body1<R>& B = b[k];
       m3<R> E;
       v3<R> r;
-      joint_frame(P.Rt(i), P.pt(i), B.c, B.s, B.d, E, r);
+      joint_frame(PL.Rt(i), PL.pt(i), B.c, B.s, B.d, E, r);
       sv<R> ap = xmotion(E, r, accel);
       ap.w.x += B.cw0; ap.w.y += B.cw1; ap.v.x += B.cv0; ap.v.y += B.cv1;
       const double qdd = (B.u - double(dot6(B.U, ap))) * B.Dinv;
@@ -881,10 +904,9 @@ __device__ __forceinline__ void tree_sie_body(const BucketArgs& a, int tid, doub
   }
 #pragma unroll
   for (int k = 0; k < DEPTH; ++k) {
-    const int j = leg * DEPTH + k;
-    qS.st((7 + j), inst, ql[k]);
-    vS.st((6 + j), inst, vl[k]);
-    vdotS.st((6 + j), inst, al[k]);
+    qS.stx(7 + k, inst, legrow, ql[k]);
+    vS.stx(6 + k, inst, legrow, vl[k]);
+    vdotS.stx(6 + k, inst, legrow, al[k]);
     ok = ok && isfinite(ql[k]) && isfinite(vl[k]) && isfinite(al[k]);
   }
   if (!ok) a.fault[inst] = 1;

@@ -219,11 +219,12 @@ int gp_batch_last_timing(const gp_batch* batch, double* ms, int64_t* launches);
 
 /* Autotuner. No counterpart in the reference (one mechanism per process, no
  * launch shape to choose). Times short trial rollouts of this batch (trial_steps
- * steps each, 0 = 200; the faster of two runs) while it walks, one option at a
- * time, lanes_per_mech (1/2/4/8, if a FAST free-body bucket exists),
- * lanes_per_tree (1/4, if a FAST quadruped bucket exists), block_size
- * (64/128/256) and steps_per_launch (25/50/100/200), keeping each value that is
- * faster. The batch's state, vdot and fault flags are restored afterwards and
+ * steps each, 0 = 200; the fastest of three runs, after one untimed warm-up)
+ * while it walks, one option at a time, lanes_per_mech (1/2/4/8, if a FAST
+ * free-body bucket exists), lanes_per_tree (1/4, if a FAST quadruped bucket
+ * exists), block_size (64/128/256) and steps_per_launch (25/50/100/200),
+ * keeping a value only if it is at least 2 % faster (zero options are replaced
+ * by the defaults they stand for first). The batch's state, vdot and fault flags are restored afterwards and
  * the chosen options are applied (use_graph kept as set) and written to
  * *chosen; *best_ms is the chosen trial's time. Workgroup size and steps per
  * launch do not change any result bit; the lane counts change the order of

@@ -1,0 +1,133 @@
+"""The reference's own contact and spring tests (contact.rs, dynamics.rs),
+run through the HIP path: each model is a batch of one instance stepped by
+gp_simulate with RK4 (the reference's float-accumulated loop), FP64. The
+reference's assertions are applied to the GPU trajectory, and the trajectory
+is compared with the oracle's: within 1e-9 over the first 10 steps, and over
+the whole rollout within 1e-3 on positions and 1e-2 on velocities (a tenth
+of the reference's own 1e-2 position tolerance). The regularised Coulomb
+friction (slip velocity 1e-3 m/s, contact.rs:287) and the stiff contact
+springs amplify last-bit differences between the two algorithms over
+thousands of steps; the per-step bound (1e-10) is tested elsewhere."""
+import math
+
+import numpy as np
+import pytest
+
+from test_oracle_golden import G, ball_with_point, rod_model
+
+pytestmark = pytest.mark.gpu
+
+RK4 = 2
+
+
+def _simulate(gp, st, final_time, dt, q0, v0, tau=None, integrator=RK4):
+    from gorilla_physics_amd import batch
+    m = batch.Model(st.model_desc())
+    b = batch.Batch([m], [1], precision=0)
+    b.set_state(np.asarray(q0, float), np.asarray(v0, float))
+    control = None if tau is None else (lambda q, v, t: tau)
+    return b.simulate(final_time, dt, control, integrator)
+
+
+def _same_as_oracle(oracle, st, final_time, dt, q0, v0, qs, vs, tau=None, integrator=RK4):
+    rq, rv = oracle.simulate(st.model_desc(), final_time, dt, q0, v0, tau, integrator)
+    assert qs.shape == rq.shape
+    eq, ev = np.abs(qs - rq).max(axis=1), np.abs(vs - rv).max(axis=1)
+    print(f"vs oracle: first 10 steps {max(eq[:11].max(), ev[:11].max()):.3g}, "
+          f"whole rollout q {eq.max():.3g} v {ev.max():.3g}", flush=True)
+    assert eq[:11].max() < 1e-9 and ev[:11].max() < 1e-9
+    assert eq.max() < 1e-3 and ev.max() < 1e-2
+
+
+def test_cube_fall_ground(gp, oracle):  # contact.rs:408
+    from gorilla_physics_amd import builders
+    print("case reference cube_fall_ground", flush=True)
+    st = builders.build_cube(3.0, 1.0)
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1], -10.0))
+    q0, v0 = [0, 0, 0, 1, 0, 0, 0.0], [1, 1, 1, 1, 1, 1.0]
+    qs, vs = _simulate(gp, st, 5.0, 1e-3, q0, v0)
+    assert abs(qs[-1, 6] - (-10.0 + 0.5)) < 1e-2
+    _same_as_oracle(oracle, st, 5.0, 1e-3, q0, v0, qs, vs)
+
+
+def test_cube_slide_ground(gp, oracle):  # contact.rs:444
+    from gorilla_physics_amd import builders
+    print("case reference cube_slide_ground", flush=True)
+    st = builders.build_cube(3.0, 1.0)
+    st.add_halfspace(gp.HalfSpace.new_with_params([0, 0, 1], -0.5, 1.0, 0.5))
+    q0, v0 = [0, 0, 0, 1, 0, 0, 0.0], [0, 0, 0, 1, 0, 0.0]
+    qs, vs = _simulate(gp, st, 2.0, 1e-3, q0, v0)
+    assert abs(qs[-1, 6]) < 1e-2
+    assert np.linalg.norm(vs[-1, 3:]) < 5e-3 and np.linalg.norm(vs[-1, :3]) < 1e-2
+    t = 1.0 / (G * 0.5)
+    assert abs(qs[-1, 4] - (t - 0.5 * G * 0.5 * t * t)) < 1e-2
+    _same_as_oracle(oracle, st, 2.0, 1e-3, q0, v0, qs, vs)
+
+
+def test_mass_hit_ground(gp, oracle):  # contact.rs:607
+    print("case reference mass_hit_ground", flush=True)
+    st = ball_with_point(gp, m=1.0, k=50e3, alpha=1.0, mu=0.5)
+    st.halfspaces[0] = gp.HalfSpace.new_with_params([0, 0, 1], -0.3, 1.0, 0.5)
+    q0, v0 = [0, 0, 0, 1, 0, 0, 0.0], [0, 0, 0, 2.0, 0, 0]
+    qs, vs = _simulate(gp, st, 2.0, 1e-3, q0, v0)
+    assert abs(qs[-1, 6] + 0.3) < 1e-2
+    assert np.linalg.norm(vs[-1, 3:]) < 1e-2 and np.linalg.norm(vs[-1, :3]) < 1e-3
+    th = math.sqrt(0.3 * 2 / G)
+    vx = 2.0 - G * th * 0.5
+    ts = vx / (G * 0.5)
+    assert abs(qs[-1, 4] - (th * 2.0 + vx * ts - 0.5 * G * 0.5 * ts * ts)) < 1e-2
+    _same_as_oracle(oracle, st, 2.0, 1e-3, q0, v0, qs, vs)
+
+
+def test_pendulum_hit_ground(gp, oracle):  # contact.rs:371
+    print("case reference pendulum_hit_ground", flush=True)
+    m, l = 1.5, 10.0
+    st = rod_model(gp, m, l, gp.Isometry3.identity(), [0, 1, 0], cross=[m * l, 0, 0],
+                   moment=np.diag([0, m * l * l, m * l * l]))
+    st.add_contact_point(gp.ContactPoint.new("rod", [l, 0, 0]))
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1], -5.0))
+    qs, vs = _simulate(gp, st, 5.0, 1e-2, [0.0], [0.0], tau=[0.0])
+    assert abs(qs[-1, 0] - math.radians(30)) < 1e-3
+    _same_as_oracle(oracle, st, 5.0, 1e-2, [0.0], [0.0], qs, vs, tau=[0.0])
+
+
+def test_spring_on_frictionless_ground(gp, oracle):  # dynamics.rs:1133
+    print("case reference spring on frictionless ground", flush=True)
+    A = gp.RigidBody.new_sphere(1.0, 0.1, "A")
+    B = gp.RigidBody.new_sphere(1.0, 0.1, "B")
+    st = gp.MechanismState([gp.FloatingJoint(gp.Transform3D.identity("A", gp.WORLD_FRAME)),
+                            gp.PrismaticJoint.new_with_spring(gp.Transform3D.identity("B", "A"), [1, 0, 0],
+                                                              gp.JointSpring(50.0, 2.0 / 3.0))], [A, B])
+    st.add_halfspace(gp.HalfSpace.new_with_params([0, 0, 1], 0.0, 1.0, 0.0))
+    q0, v0 = [0, 0, 0, 1, 0, 0, 0, 2.0], np.zeros(7)
+    qs, vs = _simulate(gp, st, 1.1, 1e-3, q0, v0)
+    xA = qs[-1, 4]
+    xB = xA + qs[-1, 7]
+    assert abs((xA + xB) / 2 - 1.0) < 1e-5 and xA > 0 and xB < 2.0
+    _same_as_oracle(oracle, st, 1.1, 1e-3, q0, v0, qs, vs)
+
+
+def test_spring_drop(gp, oracle):  # contact.rs:908
+    """A floating sphere with a sprung prismatic leg (k = 100, rest 0) dropped
+    onto the ground (alpha 1, mu 0): the foot settles at the ground within twice
+    the static penetration and the body stays upright."""
+    print("case reference spring_drop", flush=True)
+    m_body = m_foot = 1.0
+    l_leg, h_ground = 1.0, -2.0
+    body = gp.RigidBody.new_sphere(m_body, 1.0, "body")
+    foot = gp.RigidBody.new_sphere(m_foot, 1.0, "foot")
+    st = gp.MechanismState([gp.FloatingJoint(gp.Transform3D.identity("body", gp.WORLD_FRAME)),
+                            gp.PrismaticJoint.new_with_spring(
+                                gp.Transform3D("foot", "body", gp.Isometry3.translation_xyz(0.0, 0.0, -l_leg)),
+                                [0, 0, -1], gp.JointSpring(100.0, 0.0))], [body, foot])
+    st.add_contact_point(gp.ContactPoint.new("foot", [0, 0, 0]))
+    st.add_halfspace(gp.HalfSpace.new_with_params([0, 0, 1], h_ground, 1.0, 0.0))
+    q0, v0 = [0, 0, 0, 1, 0, 0, 0, 0.0], np.zeros(7)
+    qs, vs = _simulate(gp, st, 4.0, 1e-3, q0, v0)
+    # foot position: body translation + R_body (0, 0, -l_leg - q_spring)
+    zb, qspring = qs[-1, 6], qs[-1, 7]
+    z_foot = zb - l_leg - qspring
+    z_tol = ((m_body + m_foot) * G / 50e3) ** (2.0 / 3.0)
+    assert abs(z_foot - h_ground) < 2 * z_tol
+    assert np.allclose(qs[-1, :3], 0.0, atol=1e-12) and abs(qs[-1, 3] - 1.0) < 1e-12  # upright
+    _same_as_oracle(oracle, st, 4.0, 1e-3, q0, v0, qs, vs)
