@@ -227,14 +227,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (0 = the workload's own: arms and mixed "
                                                             "1000, cubes 5000, acrobots 2000)")
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=-1, help="untimed steps (default: one launch's worth)")
     ap.add_argument("--workload", default="arms")
     ap.add_argument("--instances", type=int, default=0)
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
-    ap.add_argument("--autotune", type=int, default=1, help="gp_batch_autotune before the warmup")
+    ap.add_argument("--autotune", type=int, default=-1,
+                    help="gp_batch_autotune before the warmup (1), or fixed options (0); default: on for the "
+                         "multi-bucket workloads (mixed, cubes), off for the arms and acrobots, which run the "
+                         "options the tuner picks for them (200 steps per launch, 256-lane workgroups) so that "
+                         "a profiler run of the same command sees the same launches")
     ap.add_argument("--lanes-tree", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
@@ -274,7 +278,15 @@ def main():
     log(f"workload {args.workload}: {sum(w['counts'])} instances")
     if args.steps <= 0:
         args.steps = int(w["steps"])
+    single = args.workload in ("arms", "acrobot")
+    if args.autotune < 0:
+        args.autotune = 0 if single else 1
+    if single and not args.autotune:
+        args.steps_per_launch = args.steps_per_launch or min(args.steps, 200)
+        args.block = args.block or 256
     spl = args.steps_per_launch or min(args.steps, 100)
+    if args.warmup < 0:
+        args.warmup = spl
     common = dict(block_size=args.block, lanes_per_mech=args.lanes, lanes_per_tree=args.lanes_tree,
                   steps_per_launch=spl, use_graph=args.graph, autotune=bool(args.autotune))
     mixed = run_variant(w, models, args.steps, args.warmup, precision=abi.GP_PREC_MIXED, kernel=abi.GP_KERNEL_AUTO,

@@ -136,6 +136,10 @@ SIGNATURES = {
     "gp_batch_faults": (C.c_int, [_P, C.POINTER(C.c_uint8)]),
     "gp_batch_clear_faults": (C.c_int, [_P]),
     "gp_batch_last_timing": (C.c_int, [_P, _D, C.POINTER(C.c_int64)]),
+    "gp_batch_set_state_dev": (C.c_int, [_P, C.c_void_p, C.c_void_p, _P]),
+    "gp_batch_get_state_dev": (C.c_int, [_P, C.c_void_p, C.c_void_p, _P]),
+    "gp_batch_set_tau_dev": (C.c_int, [_P, C.c_void_p, _P]),
+    "gp_batch_get_vdot_dev": (C.c_int, [_P, C.c_void_p, _P]),
     "gp_batch_autotune": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, C.POINTER(gp_batch_opts), _D]),
     "gp_server_create": (C.c_int, [C.POINTER(gp_server_opts), C.POINTER(_P)]),
     "gp_server_destroy": (None, [_P]),

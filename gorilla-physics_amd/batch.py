@@ -164,6 +164,20 @@ class Batch:
     def clear_faults(self):
         abi.check(self.lib.gp_batch_clear_faults(self.handle), self.lib)
 
+    # device-array transfers: q, v, tau, vdot are device addresses (ints), e.g.
+    # torch tensors' data_ptr(); stream is a hipStream_t address or None
+    def set_state_dev(self, q_ptr, v_ptr, stream=None):
+        abi.check(self.lib.gp_batch_set_state_dev(self.handle, q_ptr, v_ptr, stream), self.lib)
+
+    def get_state_dev(self, q_ptr, v_ptr, stream=None):
+        abi.check(self.lib.gp_batch_get_state_dev(self.handle, q_ptr, v_ptr, stream), self.lib)
+
+    def set_tau_dev(self, tau_ptr, stream=None):
+        abi.check(self.lib.gp_batch_set_tau_dev(self.handle, tau_ptr, stream), self.lib)
+
+    def get_vdot_dev(self, vdot_ptr, stream=None):
+        abi.check(self.lib.gp_batch_get_vdot_dev(self.handle, vdot_ptr, stream), self.lib)
+
     def autotune(self, dt, integrator=abi.GP_SEMI_IMPLICIT_EULER, trial_steps=0):
         """gp_batch_autotune: time trial rollouts over lanes / workgroup size /
         steps per launch, apply the fastest options (state left unchanged).

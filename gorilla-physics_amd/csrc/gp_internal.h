@@ -124,5 +124,9 @@ int launch_fast_step(int topo_id, int precision, const BucketArgs& a, double dt,
 int launch_fast_rk_step(int topo_id, int precision, int integrator, const BucketArgs& a, double dt, int n_steps,
                         int block, void* stream);
 size_t plain_scratch_doubles(int nb, int nq, int nv, int Bp);
+// rows x B values between a bucket's SoA rows and a flat instance-major array
+// (lane l's values start at flat + flat_off[l]); to_soa = 1: flat -> SoA.
+int launch_flat_xfer(double* soa, int Bp, int B, int rows, const int64_t* flat_off, double* flat, int to_soa,
+                     void* stream);
 
 }  // namespace gp

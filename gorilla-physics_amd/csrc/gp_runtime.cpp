@@ -239,10 +239,11 @@ int model_create(const gp_model_desc* d, gp_model** out) {
 Bucket::~Bucket() { release(); }
 
 void Bucket::release() {
-  void* ptrs[] = {d_arena, d_scratch, d_active};
+  void* ptrs[] = {d_arena, d_scratch, d_active, d_flat_off};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   d_arena = nullptr;
+  d_flat_off = nullptr;
   d_parent = d_qoff = d_voff = d_cstart = d_kind = d_fault = d_active = nullptr;
   d_bp = d_cp = d_hp = d_q = d_v = d_tau = d_tau_tmp = d_vdot = d_scratch = nullptr;
   d_bpf = d_cpf = d_hpf = nullptr;
@@ -564,6 +565,16 @@ int gp_batch_create(gp_model* const* models, const int64_t* counts, int32_t n_mo
       bt->qoff[i + 1] = bt->qoff[i] + b.nq;
       bt->voff[i + 1] = bt->voff[i] + b.nv;
     }
+    // each lane's offsets into the flattened arrays, for the device-array calls
+    for (Bucket& b : bt->buckets) {
+      std::vector<int64_t> off(2 * (size_t)b.B);
+      for (int l = 0; l < b.B; ++l) {
+        off[l] = bt->qoff[b.inst[l]];
+        off[b.B + l] = bt->voff[b.inst[l]];
+      }
+      HIP_TRY(hipMalloc((void**)&b.d_flat_off, off.size() * sizeof(int64_t)));
+      HIP_TRY(hipMemcpy(b.d_flat_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipStreamCreate(&bt->stream));
     HIP_TRY(hipEventCreate(&bt->ev0));
     HIP_TRY(hipEventCreate(&bt->ev1));
@@ -656,6 +667,43 @@ static int transfer(gp_batch* bt, const double* hq, const double* hv, double* oq
     }
   }
   return GP_OK;
+}
+
+// device arrays <-> bucket SoA on `stream` (which = 0 state, 1 tau, 2 vdot)
+static int transfer_dev(gp_batch* bt, const double* q, const double* v, double* oq, double* ov, int which,
+                        void* stream) {
+  hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
+  const bool to_dev = oq == nullptr && ov == nullptr;
+  for (Bucket& b : bt->buckets) {
+    int e = 0;
+    if (which == 0)
+      e = launch_flat_xfer(b.d_q, b.Bp, b.B, b.nq, b.d_flat_off, to_dev ? const_cast<double*>(q) : oq, to_dev, s);
+    if (e) return fail(GP_ERR_DEVICE, "transposition launch: %s", hipGetErrorString((hipError_t)e));
+    double* dv = which == 0 ? b.d_v : which == 1 ? b.d_tau : b.d_vdot;
+    e = launch_flat_xfer(dv, b.Bp, b.B, b.nv, b.d_flat_off + b.B, to_dev ? const_cast<double*>(v) : ov, to_dev, s);
+    if (e) return fail(GP_ERR_DEVICE, "transposition launch: %s", hipGetErrorString((hipError_t)e));
+  }
+  return GP_OK;
+}
+
+int gp_batch_set_state_dev(gp_batch* bt, const double* q, const double* v, void* stream) {
+  if (!bt || !q || !v) return fail(GP_ERR_INVALID, "null argument");
+  return transfer_dev(bt, q, v, nullptr, nullptr, 0, stream);
+}
+
+int gp_batch_get_state_dev(gp_batch* bt, double* q, double* v, void* stream) {
+  if (!bt || !q || !v) return fail(GP_ERR_INVALID, "null argument");
+  return transfer_dev(bt, nullptr, nullptr, q, v, 0, stream);
+}
+
+int gp_batch_set_tau_dev(gp_batch* bt, const double* tau, void* stream) {
+  if (!bt || !tau) return fail(GP_ERR_INVALID, "null argument");
+  return transfer_dev(bt, nullptr, tau, nullptr, nullptr, 1, stream);
+}
+
+int gp_batch_get_vdot_dev(gp_batch* bt, double* vdot, void* stream) {
+  if (!bt || !vdot) return fail(GP_ERR_INVALID, "null argument");
+  return transfer_dev(bt, nullptr, nullptr, nullptr, vdot, 2, stream);
 }
 
 int gp_batch_set_state(gp_batch* bt, const double* q, const double* v) {

@@ -44,6 +44,7 @@ struct Bucket {
   double* d_scratch = nullptr;
   void* d_arena = nullptr;  // one allocation holding every array above except scratch and active
   size_t arena_bytes = 0;
+  int64_t* d_flat_off = nullptr;  // [2][B]: each lane's q and v offsets in the flattened arrays
   Bucket() = default;
   Bucket(const Bucket&) = delete;
   Bucket& operator=(const Bucket&) = delete;
@@ -59,7 +60,9 @@ struct Bucket {
       d_bp = o.d_bp; d_cp = o.d_cp; d_hp = o.d_hp; d_bpf = o.d_bpf; d_cpf = o.d_cpf; d_hpf = o.d_hpf;
       d_q = o.d_q; d_v = o.d_v; d_tau = o.d_tau; d_tau_tmp = o.d_tau_tmp; d_vdot = o.d_vdot; d_fault = o.d_fault;
       d_active = o.d_active; d_scratch = o.d_scratch; d_arena = o.d_arena; arena_bytes = o.arena_bytes;
+      d_flat_off = o.d_flat_off;
       o.d_arena = nullptr;
+      o.d_flat_off = nullptr;
       o.d_parent = o.d_qoff = o.d_voff = o.d_cstart = o.d_kind = o.d_fault = o.d_active = nullptr;
       o.d_bp = o.d_cp = o.d_hp = o.d_q = o.d_v = o.d_tau = o.d_tau_tmp = o.d_vdot = o.d_scratch = nullptr;
       o.d_bpf = o.d_cpf = o.d_hpf = nullptr;

@@ -178,6 +178,17 @@ int gp_batch_get_state(gp_batch* batch, double* q, double* v);
  * simulate.rs:20); NULL = zeros, as an empty tau in simulate.rs:27-37. */
 int gp_batch_set_tau(gp_batch* batch, const double* tau);
 
+/* The same transfers with DEVICE arrays in the same instance-major flattening,
+ * asynchronous on `stream` (a hipStream_t; NULL = the batch's stream): the
+ * transposition to and from the buckets' structure-of-arrays runs on the GPU.
+ * For callers whose state or torques already live in HBM (a policy network's
+ * output, a tensor), so a control loop never crosses PCIe. No reference
+ * counterpart (the reference's state is host memory). */
+int gp_batch_set_state_dev(gp_batch* batch, const double* q, const double* v, void* stream);
+int gp_batch_get_state_dev(gp_batch* batch, double* q, double* v, void* stream);
+int gp_batch_set_tau_dev(gp_batch* batch, const double* tau, void* stream);
+int gp_batch_get_vdot_dev(gp_batch* batch, double* vdot, void* stream);
+
 /* step(state, dt, tau, integrator) (simulate.rs:20) applied n_steps times with
  * the held torques. Asynchronous on `stream` (a hipStream_t; NULL = the batch's
  * own stream). */
