@@ -11,7 +11,8 @@ The directory name is not a Python identifier; load it with `load()` below
 from . import abi  # noqa: F401
 from .mechanism import (GRAVITY, WORLD_FRAME, ContactPoint, FixedJoint, FloatingJoint, HalfSpace,  # noqa: F401
                         Integrator, Isometry3, JointSpring, MechanismState, Pose, PrismaticJoint, RevoluteJoint,
-                        RigidBody, SpatialInertia, SpatialVector, Transform3D, UnitQuaternion, dynamics_continuous,
+                        RigidBody, SpatialInertia, SpatialVector, SpringContact, Transform3D, UnitQuaternion,
+                        dynamics_continuous,
                         simulate, step)
 
 

@@ -64,6 +64,11 @@ class gp_halfspace_desc(C.Structure):
     ]
 
 
+class gp_spring_contact_desc(C.Structure):
+    _fields_ = [("body", C.c_int32), ("_pad", C.c_int32), ("direction", C.c_double * 3), ("l_rest", C.c_double),
+                ("k", C.c_double)]
+
+
 class gp_model_desc(C.Structure):
     _fields_ = [
         ("n_bodies", C.c_int32),
@@ -74,6 +79,9 @@ class gp_model_desc(C.Structure):
         ("contacts", C.POINTER(gp_contact_desc)),
         ("halfspaces", C.POINTER(gp_halfspace_desc)),
         ("gravity", C.c_double),
+        ("n_spring_contacts", C.c_int32),
+        ("_pad2", C.c_int32),
+        ("spring_contacts", C.POINTER(gp_spring_contact_desc)),
     ]
 
 
@@ -140,6 +148,8 @@ SIGNATURES = {
     "gp_batch_get_state_dev": (C.c_int, [_P, C.c_void_p, C.c_void_p, _P]),
     "gp_batch_set_tau_dev": (C.c_int, [_P, C.c_void_p, _P]),
     "gp_batch_get_vdot_dev": (C.c_int, [_P, C.c_void_p, _P]),
+    "gp_batch_set_spring_contact": (C.c_int, [_P, C.c_int64, C.c_int32, _D, C.c_double]),
+    "gp_batch_get_spring_contact": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _D, _D, _D]),
     "gp_batch_autotune": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, C.POINTER(gp_batch_opts), _D]),
     "gp_server_create": (C.c_int, [C.POINTER(gp_server_opts), C.POINTER(_P)]),
     "gp_server_destroy": (None, [_P]),

@@ -178,6 +178,20 @@ class Batch:
     def get_vdot_dev(self, vdot_ptr, stream=None):
         abi.check(self.lib.gp_batch_get_vdot_dev(self.handle, vdot_ptr, stream), self.lib)
 
+    def set_spring_contact(self, instance, index, direction, l_rest):
+        """gp_batch_set_spring_contact: direction (body frame) and rest length."""
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        abi.check(self.lib.gp_batch_set_spring_contact(self.handle, int(instance), int(index), _dp(d),
+                                                       float(l_rest)), self.lib)
+
+    def get_spring_contact(self, instance, index):
+        """(registered, world point, direction in the body frame, rest length)."""
+        reg = C.c_int32()
+        p, d, lr = np.zeros(3), np.zeros(3), C.c_double()
+        abi.check(self.lib.gp_batch_get_spring_contact(self.handle, int(instance), int(index), C.byref(reg), _dp(p),
+                                                       _dp(d), C.byref(lr)), self.lib)
+        return bool(reg.value), p, d, lr.value
+
     def autotune(self, dt, integrator=abi.GP_SEMI_IMPLICIT_EULER, trial_steps=0):
         """gp_batch_autotune: time trial rollouts over lanes / workgroup size /
         steps per launch, apply the fastest options (state left unchanged).

@@ -3,7 +3,7 @@ arguments), used by the tests and the workload generators."""
 import numpy as np
 
 from .mechanism import (WORLD_FRAME, ContactPoint, FloatingJoint, Isometry3, MechanismState, PrismaticJoint,
-                        RevoluteJoint, RigidBody, SpatialInertia, Transform3D)
+                        RevoluteJoint, RigidBody, SpatialInertia, SpringContact, Transform3D)
 
 
 def build_pendulum(mass, moment, cross_part, rod_to_world, axis):  # helpers.rs:24
@@ -44,6 +44,16 @@ def build_cube(mass, length):  # helpers.rs:151
     cube = RigidBody.new_cube(mass, length, "cube")
     state = MechanismState([FloatingJoint(Transform3D.identity("cube", WORLD_FRAME))], [cube])
     add_cube_contacts(state, "cube", length)
+    return state
+
+
+def build_SLIP(m, r, l_rest, angle, k_spring):  # helpers.rs:308
+    """Spring-loaded inverted pendulum: a floating solid sphere (moment
+    2/5 m r^2) with one spring leg at `angle` from straight down, forward."""
+    mx = 2.0 / 5.0 * m * r * r
+    body = RigidBody(SpatialInertia("body", np.diag([mx, mx, mx]), np.zeros(3), m))
+    state = MechanismState([FloatingJoint(Transform3D.identity("body", WORLD_FRAME))], [body])
+    state.add_spring_contact(SpringContact("body", l_rest, [np.sin(angle), 0.0, -np.cos(angle)], k_spring))
     return state
 
 

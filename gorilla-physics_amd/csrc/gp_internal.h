@@ -33,6 +33,12 @@ enum BodyField {
 };
 enum ContactField { CF_P = 0, CF_K = 3, NCF = 4 };
 enum HalfspaceField { HF_POINT = 0, HF_NORMAL = 3, HF_ALPHA = 6, HF_MU = 7, NHF = 8 };
+// spring contacts: parameters (direction in the aligned body frame, rest
+// length, stiffness, A^T of the body's frame alignment, row-major, to turn a
+// world direction into the stored one) and registration state (registered
+// flag, registered world point, half-space index); both [field][nsp][Bp]
+enum SpringField { SF_DIR = 0, SF_L = 3, SF_K = 4, SF_AT = 5, NSF = 14 };
+enum SpringState { SS_REG = 0, SS_P = 1, SS_H = 4, NSS = 5 };
 
 // per-lane scratch fields of the runtime-topology (plain) kernel, per body
 enum PlainField {
@@ -77,6 +83,10 @@ struct BucketArgs {
   const double* cp;    // [NCF][nc][Bp]
   const float* cpf;
   const double* hp;    // [NHF][nh][Bp]
+  int nsp;             // spring contacts per instance
+  const int* sstart;   // [nb + 1] spring contacts sorted by body
+  double* sp;          // [NSF][nsp][Bp] (direction and rest length change when a leg detaches)
+  double* ss;          // [NSS][nsp][Bp]
   const float* hpf;
   // state
   double* q;           // [nq][Bp]

@@ -104,7 +104,7 @@ __device__ bool plain_dynamics(const BucketArgs& a, int lane, Strided q, Strided
     v3<double> cm{X.bpar(BF_C, i), X.bpar(BF_C + 1, i), X.bpar(BF_C + 2, i)};
     ai6<double> I = rigid(J, cm, X.bpar(BF_M, i));
     sv<double> f = crf(vel, mul(I, vel));
-    if (a.nc > 0) {
+    if (a.nc > 0 || a.nsp > 0) {
       m3<double> Rw = Rm;
       v3<double> tw = p;
       if (pr >= 0) {
@@ -130,6 +130,46 @@ __device__ bool plain_dynamics(const BucketArgs& a, int lane, Strided q, Strided
             v3<double> Fb = mulT(Rw, F);
             f.w = f.w - cross(pb, Fb);
             f.v = f.v - Fb;
+          }
+        }
+      }
+      // spring contacts (contact.rs:131-186); their registration changes here,
+      // once per dynamics evaluation, as in the reference
+      for (int sc = a.sstart[i]; sc < a.sstart[i + 1]; ++sc) {
+        auto spar = [&](int fld) -> double& { return a.sp[((size_t)fld * a.nsp + sc) * Bp + lane]; };
+        auto sst = [&](int fld) -> double& { return a.ss[((size_t)fld * a.nsp + sc) * Bp + lane]; };
+        auto hpar = [&](int h, int fld) { return a.hp[((size_t)fld * a.nh + h) * Bp + lane]; };
+        if (sst(SS_REG) == 0.0) {
+          // the leg's tip: body origin + R direction l_rest; the first half-space holding it registers it
+          const v3<double> tip = tw + spar(SF_L) * mul(Rw, v3<double>{spar(SF_DIR), spar(SF_DIR + 1), spar(SF_DIR + 2)});
+          for (int h = 0; h < a.nh; ++h) {
+            const v3<double> pt_h{hpar(h, HF_POINT), hpar(h, HF_POINT + 1), hpar(h, HF_POINT + 2)};
+            const v3<double> n{hpar(h, HF_NORMAL), hpar(h, HF_NORMAL + 1), hpar(h, HF_NORMAL + 2)};
+            if (!(dot(tip - pt_h, n) <= kInsideMargin)) continue;
+            sst(SS_REG) = 1.0;
+            sst(SS_P) = tip.x; sst(SS_P + 1) = tip.y; sst(SS_P + 2) = tip.z;
+            sst(SS_H) = (double)h;
+            break;
+          }
+        } else {
+          const v3<double> diff = v3<double>{sst(SS_P), sst(SS_P + 1), sst(SS_P + 2)} - tw;
+          const double dist = sqrt(dot(diff, diff));
+          const v3<double> sdir{diff.x / dist, diff.y / dist, diff.z / dist};
+          const int h = (int)sst(SS_H);
+          const v3<double> n{hpar(h, HF_NORMAL), hpar(h, HF_NORMAL + 1), hpar(h, HF_NORMAL + 2)};
+          if (dot(sdir, n) > 0.0) ok = false;  // a pull into the half-space: the reference panics
+          if (dist < spar(SF_L)) {
+            const double force = -spar(SF_K) * (dist - spar(SF_L));  // energy_control.rs:13
+            const v3<double> F = force * (-sdir);                    // at the body's origin: no torque about it
+            f.v = f.v - mulT(Rw, F);
+          } else {
+            // detached: the leg keeps its last direction, a world vector stored as
+            // the body-frame direction (contact.rs:176-181), here in the aligned frame
+            sst(SS_REG) = 0.0;
+            const double sn = sqrt(dot(sdir, sdir));
+            const v3<double> w{sdir.x / sn, sdir.y / sn, sdir.z / sn};
+            for (int r = 0; r < 3; ++r)
+              spar(SF_DIR + r) = spar(SF_AT + 3 * r) * w.x + spar(SF_AT + 3 * r + 1) * w.y + spar(SF_AT + 3 * r + 2) * w.z;
           }
         }
       }

@@ -220,6 +220,41 @@ int model_create(const gp_model_desc* d, gp_model** out) {
     H[HF_ALPHA] = hd.alpha;
     H[HF_MU] = hd.mu;
   }
+  // spring contacts (contact.rs:74-94), stably sorted by body, directions in
+  // the aligned body frames; A^T kept to store a world direction the same way
+  if (d->n_spring_contacts < 0 || (d->n_spring_contacts > 0 && !d->spring_contacts))
+    return fail(GP_ERR_INVALID, "bad spring contacts");
+  std::vector<int> sorder(d->n_spring_contacts);
+  for (int c = 0; c < d->n_spring_contacts; ++c) {
+    const gp_spring_contact_desc& sd = d->spring_contacts[c];
+    if (sd.body < 0 || sd.body >= n) return fail(GP_ERR_INVALID, "spring contact %d: body %d out of range", c, sd.body);
+    if (!finite_all(sd.direction, 3) || !std::isfinite(sd.l_rest) || !std::isfinite(sd.k) || sd.l_rest < 0)
+      return fail(GP_ERR_INVALID, "spring contact %d: bad parameters", c);
+    const double dn = std::sqrt(sd.direction[0] * sd.direction[0] + sd.direction[1] * sd.direction[1] +
+                                sd.direction[2] * sd.direction[2]);
+    if (std::fabs(dn - 1.0) > 1e-6) return fail(GP_ERR_INVALID, "spring contact %d: direction is not a unit vector", c);
+    sorder[c] = c;
+  }
+  std::stable_sort(sorder.begin(), sorder.end(),
+                   [&](int x, int y) { return d->spring_contacts[x].body < d->spring_contacts[y].body; });
+  m->nsp = d->n_spring_contacts;
+  m->sstart.assign(n + 1, 0);
+  m->sp.assign((size_t)m->nsp * NSF, 0.0);
+  m->spring_slot.assign(m->nsp, 0);
+  for (int s = 0; s < m->nsp; ++s) {
+    const gp_spring_contact_desc& sd = d->spring_contacts[sorder[s]];
+    m->spring_slot[sorder[s]] = s;
+    m->sstart[sd.body + 1]++;
+    double* S = &m->sp[(size_t)s * NSF];
+    double dir[3];
+    mv(mT(Al[sd.body]), sd.direction, dir);
+    for (int k = 0; k < 3; ++k) S[SF_DIR + k] = dir[k];
+    S[SF_L] = sd.l_rest;
+    S[SF_K] = sd.k;
+    const M At = mT(Al[sd.body]);
+    for (int k = 0; k < 9; ++k) S[SF_AT + k] = At.a[k];
+  }
+  for (int i = 0; i < n; ++i) m->sstart[i + 1] += m->sstart[i];
   // topology signature: bodies, parents, DOF class, contacts per body, half-spaces
   uint64_t h = 1469598103934665603ull;
   h = fnv(h, n);
@@ -227,10 +262,12 @@ int model_create(const gp_model_desc* d, gp_model** out) {
     h = fnv(h, m->parent[i]);
     h = fnv(h, m->dofcls[i]);
     h = fnv(h, m->cstart[i + 1] - m->cstart[i]);
+    h = fnv(h, m->sstart[i + 1] - m->sstart[i]);
   }
   h = fnv(h, m->nh);
   m->sig = h;
-  m->fast_topo = fast_topology_id(n, m->parent.data(), m->dofcls.data(), m->nc);
+  // spring contacts hold per-instance state the compiled-topology kernels do not carry
+  m->fast_topo = m->nsp > 0 ? -1 : fast_topology_id(n, m->parent.data(), m->dofcls.data(), m->nc);
   *out = m.release();
   return GP_OK;
 }
@@ -244,6 +281,8 @@ void Bucket::release() {
     if (p) (void)hipFree(p);
   d_arena = nullptr;
   d_flat_off = nullptr;
+  d_sstart = nullptr;
+  d_sp = d_ss = nullptr;
   d_parent = d_qoff = d_voff = d_cstart = d_kind = d_fault = d_active = nullptr;
   d_bp = d_cp = d_hp = d_q = d_v = d_tau = d_tau_tmp = d_vdot = d_scratch = nullptr;
   d_bpf = d_cpf = d_hpf = nullptr;
@@ -255,8 +294,9 @@ void Bucket::release() {
 // are written at admission) skips the per-lane packing: the lanes start zeroed.
 int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, bool fill_lanes) {
   const gp_model* m0 = lane_models[0];
-  b.nb = m0->n; b.nq = m0->nq; b.nv = m0->nv; b.nc = m0->nc; b.nh = m0->nh; b.g = m0->g;
+  b.nb = m0->n; b.nq = m0->nq; b.nv = m0->nv; b.nc = m0->nc; b.nh = m0->nh; b.nsp = m0->nsp; b.g = m0->g;
   b.parent = m0->parent; b.qoff = m0->qoff; b.voff = m0->voff; b.dofcls = m0->dofcls; b.cstart = m0->cstart;
+  b.sstart = m0->sstart; b.spring_slot = m0->spring_slot;
   b.fast_topo = m0->fast_topo;
   b.all_rev = true;
   b.B = (int)lane_models.size();
@@ -275,13 +315,16 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
                o_bpf = take(4 * NBF * b.nb * Bp), o_cp = take(8 * NCF * b.nc * Bp), o_cpf = take(4 * NCF * b.nc * Bp),
                o_hp = take(8 * NHF * b.nh * Bp), o_hpf = take(4 * NHF * b.nh * Bp), o_q = take(8 * nq1 * Bp),
                o_v = take(8 * nv1 * Bp), o_tau = take(8 * nv1 * Bp), o_tau_tmp = take(8 * nv1 * Bp),
-               o_vdot = take(8 * nv1 * Bp), o_fault = take(4 * Bp);
+               o_vdot = take(8 * nv1 * Bp), o_fault = take(4 * Bp), o_sstart = take(4 * (b.nb + 1)),
+               o_sp = take(8 * NSF * b.nsp * Bp), o_ss = take(8 * NSS * b.nsp * Bp);
   std::vector<char> h(at, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(h.data() + off, src, bytes); };
   put(o_parent, b.parent.data(), 4 * b.nb);
   put(o_qoff, b.qoff.data(), 4 * b.nb);
   put(o_voff, b.voff.data(), 4 * b.nb);
   put(o_cstart, b.cstart.data(), 4 * (b.nb + 1));
+  put(o_sstart, b.sstart.data(), 4 * (b.nb + 1));
+  double* sp = (double*)(h.data() + o_sp);
   int* kind = (int*)(h.data() + o_kind);
   double* bp = (double*)(h.data() + o_bp);
   float* bpf = (float*)(h.data() + o_bpf);
@@ -314,6 +357,8 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
         hp[((size_t)f * b.nh + hh) * Bp + l] = x;
         hpf[((size_t)f * b.nh + hh) * Bp + l] = (float)x;
       }
+    for (int s = 0; s < b.nsp; ++s)
+      for (int f = 0; f < NSF; ++f) sp[((size_t)f * b.nsp + s) * Bp + l] = m->sp[(size_t)s * NSF + f];
   }
   // zero state (mechanism.rs:68-88): q = 0, floating q = identity pose, v = 0, tau = 0
   double* q0 = (double*)(h.data() + o_q);
@@ -331,6 +376,7 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
   b.d_cpf = (float*)(d + o_cpf); b.d_hp = (double*)(d + o_hp); b.d_hpf = (float*)(d + o_hpf);
   b.d_q = (double*)(d + o_q); b.d_v = (double*)(d + o_v); b.d_tau = (double*)(d + o_tau);
   b.d_tau_tmp = (double*)(d + o_tau_tmp); b.d_vdot = (double*)(d + o_vdot); b.d_fault = (int*)(d + o_fault);
+  b.d_sstart = (int*)(d + o_sstart); b.d_sp = (double*)(d + o_sp); b.d_ss = (double*)(d + o_ss);
   b.arena_bytes = at;
   return GP_OK;
 }
@@ -345,7 +391,7 @@ int bucket_ensure_scratch(Bucket& b) {
 
 int64_t soa_max_lanes(const gp_model& m) {
   const int64_t per_lane = 8 * std::max<int64_t>({(int64_t)NBF * m.n, (int64_t)NCF * m.nc, (int64_t)NHF * m.nh,
-                                                   (int64_t)m.nq, (int64_t)m.nv, 1});
+                                                   (int64_t)NSF * m.nsp, (int64_t)m.nq, (int64_t)m.nv, 1});
   return std::max<int64_t>(64, ((((int64_t)1 << 31) - 1) / per_lane) / 64 * 64);
 }
 
@@ -357,6 +403,7 @@ BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
   a.kind = b.d_kind; a.bp = b.d_bp; a.bpf = b.d_bpf; a.cp = b.d_cp; a.cpf = b.d_cpf; a.hp = b.d_hp; a.hpf = b.d_hpf;
   a.q = b.d_q; a.v = b.d_v; a.tau = tau ? tau : b.d_tau; a.vdot = b.d_vdot; a.fault = b.d_fault;
   a.active = b.d_active; a.scratch = b.d_scratch;
+  a.nsp = b.nsp; a.sstart = b.d_sstart; a.sp = b.d_sp; a.ss = b.d_ss;
   return a;
 }
 
@@ -706,6 +753,57 @@ int gp_batch_get_vdot_dev(gp_batch* bt, double* vdot, void* stream) {
   return transfer_dev(bt, nullptr, nullptr, nullptr, vdot, 2, stream);
 }
 
+int gp_batch_set_spring_contact(gp_batch* bt, int64_t inst, int32_t index, const double* direction, double l_rest) {
+  if (!bt || !direction) return fail(GP_ERR_INVALID, "null argument");
+  if (inst < 0 || inst >= bt->n_inst) return fail(GP_ERR_INVALID, "instance %lld out of range", (long long)inst);
+  Bucket& b = bt->buckets[bt->inst_bucket[inst]];
+  if (index < 0 || index >= b.nsp) return fail(GP_ERR_INVALID, "spring contact %d out of range", index);
+  if (!finite_all(direction, 3) || !std::isfinite(l_rest) || l_rest < 0) return fail(GP_ERR_INVALID, "bad parameters");
+  const int s = b.spring_slot[index], l = bt->inst_lane[inst];
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  double at[9];
+  for (int k = 0; k < 9; ++k)
+    HIP_TRY(hipMemcpy(&at[k], b.d_sp + ((size_t)(SF_AT + k) * b.nsp + s) * b.Bp + l, 8, hipMemcpyDeviceToHost));
+  double vals[4];
+  for (int r = 0; r < 3; ++r) vals[r] = at[3 * r] * direction[0] + at[3 * r + 1] * direction[1] + at[3 * r + 2] * direction[2];
+  vals[3] = l_rest;
+  for (int k = 0; k < 4; ++k)
+    HIP_TRY(hipMemcpy(b.d_sp + ((size_t)(SF_DIR + k) * b.nsp + s) * b.Bp + l, &vals[k], 8, hipMemcpyHostToDevice));
+  return GP_OK;
+}
+
+int gp_batch_get_spring_contact(gp_batch* bt, int64_t inst, int32_t index, int32_t* registered, double* point,
+                                double* direction, double* l_rest) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  if (inst < 0 || inst >= bt->n_inst) return fail(GP_ERR_INVALID, "instance %lld out of range", (long long)inst);
+  Bucket& b = bt->buckets[bt->inst_bucket[inst]];
+  if (index < 0 || index >= b.nsp) return fail(GP_ERR_INVALID, "spring contact %d out of range", index);
+  const int s = b.spring_slot[index], l = bt->inst_lane[inst];
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  auto sp = [&](int f) {
+    double x = 0;
+    (void)hipMemcpy(&x, b.d_sp + ((size_t)f * b.nsp + s) * b.Bp + l, 8, hipMemcpyDeviceToHost);
+    return x;
+  };
+  auto ss = [&](int f) {
+    double x = 0;
+    (void)hipMemcpy(&x, b.d_ss + ((size_t)f * b.nsp + s) * b.Bp + l, 8, hipMemcpyDeviceToHost);
+    return x;
+  };
+  if (registered) *registered = ss(SS_REG) != 0.0 ? 1 : 0;
+  if (point)
+    for (int k = 0; k < 3; ++k) point[k] = ss(SS_P + k);
+  if (direction) {  // back to the reference's body frame: A d (A^T stored row-major)
+    double at[9], da[3];
+    for (int k = 0; k < 9; ++k) at[k] = sp(SF_AT + k);
+    for (int k = 0; k < 3; ++k) da[k] = sp(SF_DIR + k);
+    for (int r = 0; r < 3; ++r) direction[r] = at[r] * da[0] + at[3 + r] * da[1] + at[6 + r] * da[2];
+  }
+  if (l_rest) *l_rest = sp(SF_L);
+  HIP_TRY(hipGetLastError());
+  return GP_OK;
+}
+
 int gp_batch_set_state(gp_batch* bt, const double* q, const double* v) {
   if (!bt || !q || !v) return fail(GP_ERR_INVALID, "null argument");
   return transfer(bt, q, v, nullptr, nullptr, true, 0);
@@ -727,6 +825,10 @@ int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* 
   if (integrator < GP_SEMI_IMPLICIT_EULER || integrator > GP_RUNGE_KUTTA_4)
     return fail(GP_ERR_INVALID, "unknown integrator %d", integrator);
   if (n_steps < 0) return fail(GP_ERR_INVALID, "n_steps < 0");
+  if (integrator != GP_SEMI_IMPLICIT_EULER)
+    for (const Bucket& b : bt->buckets)
+      if (b.nsp > 0)  // simulate.rs:56-71 asserts the same
+        return fail(GP_ERR_UNSUPPORTED, "Runge-Kutta on a state with spring contacts");
   hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
   int64_t launches = 0;
   if (grouped_count(bt, integrator) > 0) {

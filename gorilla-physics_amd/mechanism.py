@@ -201,6 +201,22 @@ class ContactPoint:
         return ContactPoint(frame, location, k)
 
 
+class SpringContact:
+    """A stateful spring leg from a body's origin (contact.rs:74-94): direction
+    in the body frame (unit), rest length, stiffness."""
+
+    def __init__(self, frame, l_rest, direction, k):
+        self.frame = frame
+        self.l_rest = float(l_rest)
+        d = np.asarray(direction, dtype=float)
+        self.direction = d / np.linalg.norm(d)
+        self.k = float(k)
+
+    @staticmethod
+    def new(frame, l_rest, direction, k):
+        return SpringContact(frame, l_rest, direction, k)
+
+
 class HalfSpace:
     def __init__(self, point, normal, alpha=0.9, mu=0.5):  # halfspace.rs:6-36
         self.point = np.asarray(point, dtype=float).copy()
@@ -226,6 +242,7 @@ class RigidBody:
     def __init__(self, inertia):
         self.inertia = inertia
         self.contact_points = []
+        self.spring_contacts = []
 
     @staticmethod
     def new(inertia):
@@ -254,6 +271,11 @@ class RigidBody:
         if self.inertia.frame != cp.frame:
             raise ValueError(f"Contact point frame {cp.frame} does not match body frame {self.inertia.frame}")
         self.contact_points.append(cp)
+
+    def add_spring_contact(self, sc):  # rigid_body.rs:266
+        if self.inertia.frame != sc.frame:
+            raise ValueError(f"Spring contact point frame {sc.frame} does not match body frame {self.inertia.frame}")
+        self.spring_contacts.append(sc)
 
 
 # --------------------------------------------------------------------- joints
@@ -310,13 +332,15 @@ class Integrator:  # integrators.rs:17
 class ModelDesc:
     """Owner of the POD arrays behind one gp_model_desc."""
 
-    def __init__(self, bodies, contacts, halfspaces, gravity=GRAVITY):
+    def __init__(self, bodies, contacts, halfspaces, gravity=GRAVITY, spring_contacts=()):
         self.n_bodies = len(bodies)
         self._bodies = (abi.gp_body_desc * max(1, len(bodies)))(*bodies)
         self._contacts = (abi.gp_contact_desc * max(1, len(contacts)))(*contacts)
         self._halfspaces = (abi.gp_halfspace_desc * max(1, len(halfspaces)))(*halfspaces)
+        self._springs = (abi.gp_spring_contact_desc * max(1, len(spring_contacts)))(*spring_contacts)
         self.desc = abi.gp_model_desc(len(bodies), len(contacts), len(halfspaces), 0,
-                                      self._bodies, self._contacts, self._halfspaces, gravity)
+                                      self._bodies, self._contacts, self._halfspaces, gravity,
+                                      len(spring_contacts), 0, self._springs)
         self.nq = sum(7 if b.jtype == abi.GP_JOINT_FLOATING else (0 if b.jtype == abi.GP_JOINT_FIXED else 1)
                       for b in bodies)
         self.nv = sum(6 if b.jtype == abi.GP_JOINT_FLOATING else (0 if b.jtype == abi.GP_JOINT_FIXED else 1)
@@ -348,6 +372,14 @@ def contact_desc(body, p, k):
     c.p[:] = [float(x) for x in p]
     c.k = float(k)
     return c
+
+
+def spring_contact_desc(body, sc):
+    d = abi.gp_spring_contact_desc()
+    d.body = body
+    d.direction[:] = [float(x) for x in sc.direction]
+    d.l_rest, d.k = sc.l_rest, sc.k
+    return d
 
 
 def halfspace_desc(h):
@@ -458,6 +490,12 @@ class MechanismState:
                 break
         self._invalidate()
 
+    def add_spring_contact(self, sc):  # mechanism.rs:394
+        for b in self.bodies:
+            if b.inertia.frame == sc.frame:
+                b.add_spring_contact(sc)
+        self._invalidate()
+
     def _invalidate(self):
         self._batch = None
         self._model = None
@@ -485,14 +523,16 @@ class MechanismState:
     # -- model description in the C ABI's terms
     def model_desc(self):
         if self._desc is None:
-            bodies, contacts = [], []
+            bodies, contacts, springs = [], [], []
             for i, (j, b) in enumerate(zip(self.treejoints, self.bodies)):
                 spring = getattr(j, "spring", None)
                 bodies.append(body_desc(self.parents[i] - 1, j.jtype, j.axis, j.transform.iso, b.inertia, spring))
                 for cp in b.contact_points:
                     contacts.append(contact_desc(i, cp.location, cp.k))
+                for sc in b.spring_contacts:
+                    springs.append(spring_contact_desc(i, sc))
             hs = [halfspace_desc(h) for h in self.halfspaces]
-            self._desc = ModelDesc(bodies, contacts, hs, self.gravity)
+            self._desc = ModelDesc(bodies, contacts, hs, self.gravity, springs)
         return self._desc
 
     def _device_batch(self, opts=None):

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GP_ABI_VERSION 1
+#define GP_ABI_VERSION 2
 
 /* ---- status codes (replace the reference's panic!/expect sites) ---------- */
 enum {
@@ -110,6 +110,22 @@ typedef struct gp_halfspace_desc {
   double mu;
 } gp_halfspace_desc;
 
+/* Stateful spring contact (SpringContact, contact.rs:74-94, 131-186): a
+ * massless leg from the body's origin along `direction` (body frame, unit) of
+ * rest length l_rest and stiffness k. While unregistered, its tip is tested
+ * against the half-spaces; the first one containing the tip registers that
+ * world point. While registered, a compressed leg pushes the body away from
+ * the point (force k (l_rest - l) along the leg, at the body's origin); once
+ * it is no longer shorter than l_rest it detaches and its direction becomes
+ * the leg's last world direction. Semi-implicit Euler only (simulate.rs:56-71). */
+typedef struct gp_spring_contact_desc {
+  int32_t body;
+  int32_t _pad;
+  double direction[3];
+  double l_rest;
+  double k;
+} gp_spring_contact_desc;
+
 typedef struct gp_model_desc {
   int32_t n_bodies;
   int32_t n_contacts;
@@ -119,6 +135,9 @@ typedef struct gp_model_desc {
   const gp_contact_desc* contacts;
   const gp_halfspace_desc* halfspaces;
   double gravity; /* 9.81 in the reference (lib.rs:39) */
+  int32_t n_spring_contacts;  /* zero-initialised descriptions have none */
+  int32_t _pad2;
+  const gp_spring_contact_desc* spring_contacts;
 } gp_model_desc;
 
 /* Options of a batch. Zero-initialised = defaults. */
@@ -177,6 +196,16 @@ int gp_batch_get_state(gp_batch* batch, double* q, double* v);
 /* Torques held over the following steps (the tau argument of step(),
  * simulate.rs:20); NULL = zeros, as an empty tau in simulate.rs:27-37. */
 int gp_batch_set_tau(gp_batch* batch, const double* tau);
+
+/* A spring contact's direction (body frame, unit) and rest length for one
+ * instance, as the reference's controllers set them between steps
+ * (SLIP_control.rs:120-121, contact.rs:887-888); and its registration state:
+ * registered (0/1), the registered world point, the current direction and rest
+ * length. index counts the model's spring contacts in description order. */
+int gp_batch_set_spring_contact(gp_batch* batch, int64_t instance, int32_t index, const double* direction,
+                                double l_rest);
+int gp_batch_get_spring_contact(gp_batch* batch, int64_t instance, int32_t index, int32_t* registered,
+                                double* point, double* direction, double* l_rest);
 
 /* The same transfers with DEVICE arrays in the same instance-major flattening,
  * asynchronous on `stream` (a hipStream_t; NULL = the batch's stream): the

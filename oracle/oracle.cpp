@@ -69,6 +69,47 @@ int oracle_dynamics(const gp_model_desc* d, const double* q, const double* v, co
   return dyn_any(M, &A, method, q, v, tt.data(), vdot);
 }
 
+// One semi-implicit Euler step (simulate.rs:20) of a model with spring
+// contacts, CRBA, FP64. state: per spring contact in body order (description
+// order within a body) 9 values [registered, point x y z, half-space, direction
+// x y z (body frame), rest length], read and written (the reference mutates
+// its spring contacts inside the dynamics, contact.rs:131-186). Returns 0, 1
+// (singular mass matrix) or 2 (a spring pulling into its half-space).
+int oracle_step_spring(const gp_model_desc* d, const double* q, const double* v, const double* tau, double dt,
+                       double* q_out, double* v_out, double* vdot_out, double* state) {
+  Model<double> M = load_model<double>(*d);
+  std::vector<SpringSt<double>> ss;
+  size_t k = 0;
+  for (int i = 0; i < M.n; ++i)
+    for (size_t s = 0; s < M.springs[i].size(); ++s, ++k) {
+      const double* x = state + 9 * k;
+      SpringSt<double> st;
+      st.registered = x[0] != 0.0;
+      st.point = V3<double>(x[1], x[2], x[3]);
+      st.halfspace = int(x[4]);
+      st.dir = V3<double>(x[5], x[6], x[7]);
+      st.l_rest = x[8];
+      ss.push_back(st);
+    }
+  std::vector<double> tt(M.nv, 0.0), qo(M.nq), vo(M.nv), ao(M.nv);
+  if (tau) for (int j = 0; j < M.nv; ++j) tt[j] = tau[j];
+  auto dyn = [&](const Model<double>& MM, const double* qq, const double* vv, const double* ta, double* out) {
+    return crba_dynamics(MM, qq, vv, ta, out, (double*)nullptr, (double*)nullptr, ss.data());
+  };
+  int st = step_integrate(M, GP_SEMI_IMPLICIT_EULER, q, v, tt.data(), dt, qo.data(), vo.data(), ao.data(), dyn);
+  for (int j = 0; j < M.nq; ++j) q_out[j] = qo[j];
+  for (int j = 0; j < M.nv; ++j) { v_out[j] = vo[j]; if (vdot_out) vdot_out[j] = ao[j]; }
+  for (size_t j = 0; j < ss.size(); ++j) {
+    double* x = state + 9 * j;
+    x[0] = ss[j].registered ? 1.0 : 0.0;
+    x[1] = ss[j].point.x; x[2] = ss[j].point.y; x[3] = ss[j].point.z;
+    x[4] = double(ss[j].halfspace);
+    x[5] = ss[j].dir.x; x[6] = ss[j].dir.y; x[7] = ss[j].dir.z;
+    x[8] = ss[j].l_rest;
+  }
+  return st;
+}
+
 // mass matrix (mechanism.rs:637) and bias c(q,v) - tau_contact (dynamics.rs:233)
 int oracle_mass_matrix(const gp_model_desc* d, const double* q, const double* v, double* Mout, double* cout) {
   Model<double> M = load_model<double>(*d);

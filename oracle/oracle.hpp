@@ -152,6 +152,21 @@ struct Contact {
   V3<T> p;
   T k;
 };
+// SpringContact (contact.rs:74-94): its description values; the mutable part
+// (registration, direction, rest length) lives in SpringSt, owned by the caller
+template <class T>
+struct SpringC {
+  V3<T> dir;
+  T l_rest, k;
+};
+template <class T>
+struct SpringSt {
+  bool registered = false;
+  V3<T> point;       // registered contact, world frame
+  int halfspace = 0;
+  V3<T> dir;         // body frame, unit
+  T l_rest = T(0);
+};
 template <class T>
 struct HalfSpace {
   V3<T> point, normal;
@@ -168,6 +183,7 @@ struct Model {
   std::vector<Inertia<T>> I;
   std::vector<T> spring_k, spring_l;
   std::vector<std::vector<Contact<T>>> contacts;  // per body
+  std::vector<std::vector<SpringC<T>>> springs;   // per body, description order within a body
   std::vector<HalfSpace<T>> halfspaces;
   T g = T(9.81);
   std::vector<int> qoff, voff;  // per body
@@ -184,6 +200,7 @@ Model<T> load_model(const gp_model_desc& d) {
   M.g = T(d.gravity);
   M.parent.resize(M.n); M.jtype.resize(M.n); M.axis.resize(M.n); M.tree.resize(M.n); M.I.resize(M.n);
   M.spring_k.resize(M.n); M.spring_l.resize(M.n); M.contacts.resize(M.n); M.qoff.resize(M.n); M.voff.resize(M.n);
+  M.springs.resize(M.n);
   for (int i = 0; i < M.n; ++i) {
     const gp_body_desc& b = d.bodies[i];
     M.parent[i] = b.parent;
@@ -207,6 +224,11 @@ Model<T> load_model(const gp_model_desc& d) {
   for (int c = 0; c < d.n_contacts; ++c) {
     const gp_contact_desc& cd = d.contacts[c];
     M.contacts[cd.body].push_back({V3<T>(T(cd.p[0]), T(cd.p[1]), T(cd.p[2])), T(cd.k)});
+  }
+  for (int c = 0; c < d.n_spring_contacts; ++c) {
+    const gp_spring_contact_desc& sd = d.spring_contacts[c];
+    M.springs[sd.body].push_back({V3<T>(T(sd.direction[0]), T(sd.direction[1]), T(sd.direction[2])), T(sd.l_rest),
+                                  T(sd.k)});
   }
   for (int h = 0; h < d.n_halfspaces; ++h) {
     const gp_halfspace_desc& hd = d.halfspaces[h];
@@ -329,10 +351,27 @@ constexpr double kInsideMargin = 1e-8;   // halfspace.rs:42
 // ---------------------------------------------------------------- CRBA + RNEA + LU
 // Workspace-free restatement of dynamics_continuous (dynamics.rs:322-364).
 // Returns 0 on success, 1 if LU finds a zero pivot (the reference panics).
+template <class T> T springs_k(const Model<T>& M, int body, size_t s) { return M.springs[body][s].k; }
+
+// Spring contacts visited in body order (description order within a body):
+// the state index of every spring contact. ss may be null when the model has
+// none. Returns 2 where the reference panics on a spring pulling into its
+// half-space (contact.rs:161-163).
 template <class T>
 int crba_dynamics(const Model<T>& M, const T* q, const T* v, const T* tau, T* vdot,
-                  T* Mout = nullptr, T* cout = nullptr) {
+                  T* Mout = nullptr, T* cout = nullptr, SpringSt<T>* ss = nullptr) {
   const int n = M.n, nv = M.nv;
+  std::vector<SpringSt<T>> fresh;  // no caller state: the springs as described, unregistered
+  if (!ss) {
+    for (int i = 0; i < n; ++i)
+      for (const SpringC<T>& sc : M.springs[i]) {
+        SpringSt<T> st;
+        st.dir = sc.dir;
+        st.l_rest = sc.l_rest;
+        fresh.push_back(st);
+      }
+    ss = fresh.data();
+  }
   std::vector<Iso<T>> b2r(n);        // bodies_to_root (without the world entry)
   std::vector<SV<T>> jtw(n), tw(n);  // joint twists (body frame), body twists (world)
   for (int i = 0; i < n; ++i) {      // mechanism.rs:153-170
@@ -405,6 +444,38 @@ int crba_dynamics(const Model<T>& M, const T* q, const T* v, const T* tau, T* vd
         V3<T> F = contact_force(z, h.normal, vel, cp.k, T(kHalfspaceK), h.alpha, h.mu);
         wc[i].a += cross(pw, F);
         wc[i].l += F;
+      }
+    }
+    // spring contacts with half-spaces (contact.rs:131-186), state changed here
+    for (size_t s = 0; s < M.springs[i].size(); ++s) {
+      SpringSt<T>& st = *(ss++);
+      const V3<T> body_location = b2r[i].t;
+      if (!st.registered) {
+        const V3<T> spring_direction = R * st.dir;
+        const V3<T> contact_location = body_location + spring_direction * st.l_rest;
+        for (size_t h = 0; h < M.halfspaces.size(); ++h) {
+          const HalfSpace<T>& hs = M.halfspaces[h];
+          if (!(dot(contact_location - hs.point, hs.normal) <= T(kInsideMargin))) continue;  // has_inside
+          st.registered = true;
+          st.point = contact_location;
+          st.halfspace = int(h);
+          break;
+        }
+      } else {
+        const V3<T> diff = st.point - body_location;
+        const T dist = std::sqrt(dot(diff, diff));
+        const V3<T> spring_direction = diff / dist;  // UnitVector3::new_normalize
+        if (dot(spring_direction, M.halfspaces[st.halfspace].normal) > T(0)) return 2;
+        if (dist < st.l_rest) {
+          const T force = -springs_k(M, i, s) * (dist - st.l_rest);  // spring_force (energy_control.rs:13)
+          const V3<T> F = -spring_direction * force;
+          wc[i].a += cross(body_location, F);  // Wrench::from_force at the body's origin
+          wc[i].l += F;
+        } else {
+          st.registered = false;
+          const T nn = std::sqrt(dot(spring_direction, spring_direction));
+          st.dir = spring_direction / nn;
+        }
       }
     }
   }

@@ -30,6 +30,7 @@ class Oracle:
         lib.oracle_dynamics.argtypes = [P, _D, _D, _D, _D, C.c_int]
         lib.oracle_mass_matrix.argtypes = [P, _D, _D, _D, _D]
         lib.oracle_step.argtypes = [P, C.c_int, C.c_int, _D, _D, _D, C.c_double, _D, _D, _D]
+        lib.oracle_step_spring.argtypes = [P, _D, _D, _D, C.c_double, _D, _D, _D, _D]
         lib.oracle_rollout.argtypes = [C.POINTER(P), C.c_int64, _D, _D, _D, C.c_double, C.c_int, C.c_int,
                                        C.c_int64, C.c_int, _D]
         lib.oracle_simulate.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_int, _D, _D, _D, _D, _D,
@@ -74,6 +75,29 @@ class Oracle:
                                   _dp(ao))
         if st:
             raise RuntimeError("oracle: singular mass matrix")
+        return qo, vo, ao
+
+    @staticmethod
+    def spring_state(d):
+        """Initial spring-contact state of a model (unregistered, as described):
+        9 values per spring contact in body order, see oracle_step_spring."""
+        rows = []
+        sc = [d.desc.spring_contacts[i] for i in range(d.desc.n_spring_contacts)]
+        for s in sorted(sc, key=lambda s: s.body):  # stable: description order within a body
+            rows.append([0, 0, 0, 0, 0, s.direction[0], s.direction[1], s.direction[2], s.l_rest])
+        return np.array(rows, dtype=float).reshape(-1)
+
+    def step_spring(self, d, q, v, state, tau=None, dt=1e-3):
+        """One semi-implicit Euler step of a model with spring contacts; state
+        (from spring_state) is updated in place. Returns (q', v', vdot)."""
+        nq, nv = self.dims(d)
+        q = np.ascontiguousarray(q, float); v = np.ascontiguousarray(v, float)
+        t = np.zeros(nv) if tau is None else np.ascontiguousarray(tau, float)
+        qo, vo, ao = np.empty(nq), np.empty(nv), np.empty(nv)
+        st = self.lib.oracle_step_spring(self._desc(d), _dp(q), _dp(v), _dp(t), dt, _dp(qo), _dp(vo), _dp(ao),
+                                         _dp(state))
+        if st:
+            raise RuntimeError("oracle: singular mass matrix" if st == 1 else "oracle: spring pulls into the half-space")
         return qo, vo, ao
 
     def rollout(self, descs, q, v, tau, dt, steps, integrator=0, method=CRBA, threads=1, want_vdot=False):

@@ -1,6 +1,7 @@
 """The C-ABI library loads, exports every entry point the header declares, and
 validates models the way the reference panics (CPU only: no kernel runs)."""
 import ctypes as C
+import math
 import os
 import re
 
@@ -131,8 +132,8 @@ def test_struct_layouts_match_the_header(gp, tmp_path):
     """The ctypes mirrors have the sizes and field offsets that a C compiler
     gives the header's structs."""
     import subprocess
-    structs = ["gp_body_desc", "gp_contact_desc", "gp_halfspace_desc", "gp_model_desc", "gp_batch_opts",
-               "gp_server_opts"]
+    structs = ["gp_body_desc", "gp_contact_desc", "gp_halfspace_desc", "gp_spring_contact_desc", "gp_model_desc",
+               "gp_batch_opts", "gp_server_opts"]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gorilla_amd.h"', 'int main(void) {']
     for s in structs:
         py = getattr(gp.abi, s)
@@ -151,3 +152,35 @@ def test_struct_layouts_match_the_header(gp, tmp_path):
         assert int(got[s]) == C.sizeof(py), s
         for f in py._fields_:
             assert int(got[f"{s}.{f[0]}"]) == getattr(py, f[0]).offset, (s, f[0])
+
+
+def test_spring_contact_models(gp):
+    """A spring contact (contact.rs:74-94) is part of the topology signature;
+    its description is validated like the reference's constructor inputs."""
+    from gorilla_physics_amd import batch, builders
+    st = builders.build_SLIP(0.54, 0.1, 0.2, math.radians(45.0), 500.0)
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1], -0.3))
+    d = st.model_desc()
+    assert d.desc.n_spring_contacts == 1
+    sc = d.desc.spring_contacts[0]
+    assert sc.body == 0 and sc.k == 500.0 and sc.l_rest == 0.2
+    assert np.allclose(list(sc.direction), [math.sin(math.pi / 4), 0, -math.cos(math.pi / 4)])
+    m = batch.Model(d)
+    bare = builders.build_SLIP(0.54, 0.1, 0.2, math.radians(45.0), 500.0)
+    bare.bodies[0].spring_contacts.clear()
+    bare.add_halfspace(gp.HalfSpace.new([0, 0, 1], -0.3))
+    assert batch.Model(bare.model_desc()).signature() != m.signature()
+    for field, value in (("body", 1), ("l_rest", -1.0), ("k", float("nan"))):
+        bad = st.model_desc()
+        setattr(bad.desc.spring_contacts[0], field, value)
+        with pytest.raises(gp.abi.GPError) as e:
+            batch.Model(bad)
+        assert e.value.code == gp.abi.GP_ERR_INVALID, field
+    bad = st.model_desc()
+    bad.desc.spring_contacts[0].direction[2] = 0.0  # not a unit vector
+    with pytest.raises(gp.abi.GPError):
+        batch.Model(bad)
+    bad = st.model_desc()
+    bad.desc.spring_contacts = C.POINTER(gp.abi.gp_spring_contact_desc)()
+    with pytest.raises(gp.abi.GPError):
+        batch.Model(bad)

@@ -132,3 +132,80 @@ def test_spring_drop(gp, oracle):  # contact.rs:908
     assert abs(z_foot - h_ground) < 2 * z_tol
     assert np.allclose(qs[-1, :3], 0.0, atol=1e-12) and abs(qs[-1, 3] - 1.0) < 1e-12  # upright
     _same_as_oracle(oracle, st, 4.0, 1e-3, q0, v0, qs, vs)
+
+
+def test_SLIP_hopping(gp, oracle):  # contact.rs:836
+    """The reference's SLIP test through gp_step (semi-implicit Euler, one step
+    per call, FP64), three instances at different forward speeds in one batch.
+    At every apex the leg is swung back to 45 degrees forward with
+    gp_batch_set_spring_contact, as the reference's test does. Every step is
+    checked against the oracle stepped from the same state (q, v and the spring
+    contact's registration), within 1e-10; the reference's assertions are
+    applied to the GPU trajectory of the 5 m/s instance."""
+    from gorilla_physics_amd import batch, builders
+    print("case reference SLIP_hopping", flush=True)
+    m, r, l_rest, angle, k = 0.54, 0.1, 0.2, math.radians(45.0), 500.0
+    direction = np.array([math.sin(angle), 0.0, -math.cos(angle)])
+    st = builders.build_SLIP(m, r, l_rest, angle, k)
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1], -0.3))
+    d = st.model_desc()
+    n, dt, steps = 3, 1.0 / 2000.0, int(3.0 / (1.0 / 2000.0))
+    b = batch.Batch([batch.Model(d)], [n], precision=0)
+    speeds = [5.0, 3.0, 1.0]
+    q = np.tile([0, 0, 0, 1.0, 0, 0, 0], n)
+    v = np.concatenate([[0, 0, 0, s, 0, 0] for s in speeds])
+    b.set_state(q, v)
+    e0 = sum(oracle.energy(d, q[:7], v[:6]))
+    ostate = [oracle.spring_state(d) for _ in range(n)]
+    vz_prev = np.zeros(n)
+    energies, hs, worst = [], [], 0.0
+    for _ in range(steps):
+        b.step(dt, 0, 1)
+        qn, vn = b.get_state()
+        for i in range(n):
+            rq, rv, _ = oracle.step_spring(d, q[7 * i:7 * i + 7], v[6 * i:6 * i + 6], ostate[i], None, dt)
+            worst = max(worst, np.abs(qn[7 * i:7 * i + 7] - rq).max(), np.abs(vn[6 * i:6 * i + 6] - rv).max())
+            reg, p, dr, lr = b.get_spring_contact(i, 0)
+            assert reg == bool(ostate[i][0])
+            if reg:
+                assert np.abs(p - ostate[i][1:4]).max() < 1e-10
+            else:
+                assert np.abs(dr - ostate[i][5:8]).max() < 1e-10 and lr == ostate[i][8]
+            # the oracle continues from the GPU's state, as a per-step check does
+            if reg:
+                ostate[i][1:4] = p
+            else:
+                ostate[i][5:8] = dr
+            vz = vn[6 * i + 5]
+            if vz_prev[i] > 0.0 and vz <= 0.0:
+                if i == 0:
+                    energies.append(sum(oracle.energy(d, qn[:7], vn[:6])))
+                    hs.append(qn[6])
+                b.set_spring_contact(i, 0, direction, l_rest)
+                ostate[i][5:8] = direction
+                ostate[i][8] = l_rest
+            vz_prev[i] = vz
+        q, v = qn, vn
+    print(f"worst per-step difference {worst:.3g}, {len(hs)} apexes", flush=True)
+    assert worst < 1e-10
+    assert len(energies) > 2
+    for e in energies:
+        assert abs(e - e0) < 1e-2
+    assert abs(hs[-3] - hs[-2]) < 1e-3 and abs(hs[-2] - hs[-1]) < 1e-3
+    assert not b.faults().any()
+
+
+def test_spring_contacts_reject_runge_kutta(gp):
+    """Spring contacts change their state inside the dynamics evaluation (the
+    reference notes it does not work with Runge-Kutta, contact.rs:131-132): a
+    Runge-Kutta step of a batch that holds them is refused."""
+    from gorilla_physics_amd import batch, builders
+    st = builders.build_SLIP(0.54, 0.1, 0.2, math.radians(45.0), 500.0)
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1], -0.3))
+    b = batch.Batch([batch.Model(st.model_desc())], [1], precision=0)
+    b.set_state(np.array([0, 0, 0, 1.0, 0, 0, 0]), np.zeros(6))
+    for integrator in (1, RK4):
+        with pytest.raises(gp.abi.GPError) as e:
+            b.step(1e-3, integrator, 1)
+        assert e.value.code == gp.abi.GP_ERR_UNSUPPORTED
+    b.step(1e-3, 0, 1)

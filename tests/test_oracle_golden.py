@@ -379,3 +379,76 @@ def test_work_table_matches_opcount(gp, oracle):
     w = workloads.acrobot(n=2)
     c = oracle.opcount(w["models"][0], w["q"][:2], w["v"][:2], w["tau"][:2])
     assert sum(c.values()) == table["acrobot"]["flops_per_env_step"]
+
+
+# ------------------------------------------------------------------ spring contacts (contact.rs:74-186)
+def _slip_run(oracle, d, q, v, state, steps, dt, apex=None):
+    """Semi-implicit Euler steps of a spring-contact model; at each apex
+    (vertical velocity turning from positive to non-positive) apex(q, v, state)
+    is called, as contact.rs:878-889 does."""
+    vz_prev, qs = 0.0, []
+    for _ in range(steps):
+        q, v, _ = oracle.step_spring(d, q, v, state, None, dt)
+        qs.append(q)
+        if vz_prev > 0.0 and v[5] <= 0.0 and apex is not None:
+            apex(q, v, state)
+        vz_prev = v[5]
+    return np.array(qs), v
+
+
+def test_SLIP_hopping(gp, oracle):  # contact.rs:836
+    """The reference's SLIP test: energy at every apex within 1e-2 of the
+    initial energy, the hopping height settled within 1e-3, with the leg swung
+    back to 45 degrees forward at every apex. The spring pushes along a line
+    through the centre of mass, so the body never turns."""
+    from gorilla_physics_amd import builders
+    m, r, l_rest, angle, k = 0.54, 0.1, 0.2, math.radians(45.0), 500.0
+    direction = [math.sin(angle), 0.0, -math.cos(angle)]
+    st = builders.build_SLIP(m, r, l_rest, angle, k)
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1], -0.3))
+    d = st.model_desc()
+    q, v = np.array([0, 0, 0, 1.0, 0, 0, 0]), np.array([0, 0, 0, 5.0, 0, 0])
+    e0 = sum(oracle.energy(d, q, v))
+    state = oracle.spring_state(d)
+    energies, hs = [], []
+
+    def apex(q, v, s):
+        energies.append(sum(oracle.energy(d, q, v)))
+        hs.append(q[6])
+        s[5:8] = direction
+        s[8] = l_rest
+
+    qs, _ = _slip_run(oracle, d, q, v, state, int(3.0 / (1.0 / 2000.0)), 1.0 / 2000.0, apex)
+    assert len(energies) > 2
+    for e in energies:
+        assert abs(e - e0) < 1e-2
+    assert abs(hs[-3] - hs[-2]) < 1e-3 and abs(hs[-2] - hs[-1]) < 1e-3
+    assert np.abs(qs[:, :4] - [0, 0, 0, 1]).max() < 1e-12
+    assert qs[-1, 4] > 5.0  # it hops forward
+
+
+def test_spring_contact_vertical_bounce_closed_form(gp, oracle):
+    """A straight-down spring leg dropped 0.1 m onto the ground: the deepest
+    compression is the one energy balance fixes, 1/2 k d^2 - m g d = 1/2 m u^2
+    with u = sqrt(2 g 0.1); it bounces back to its starting height; the leg
+    leaves the ground pointing straight down at its full rest length."""
+    from gorilla_physics_amd import builders
+    m, l_rest, k, dt = 0.54, 0.2, 500.0, 1e-5
+    st = builders.build_SLIP(m, 0.1, l_rest, 0.0, k)
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1], -0.3))
+    d = st.model_desc()
+    state = oracle.spring_state(d)
+    q, v = np.array([0, 0, 0, 1.0, 0, 0, 0]), np.zeros(6)
+    tops, at_top = [], []
+
+    def apex(q, v, s):
+        tops.append(q[6])
+        at_top.append(s.copy())
+
+    qs, _ = _slip_run(oracle, d, q, v, state, int(0.6 / dt), dt, apex)
+    u = math.sqrt(2 * G * 0.1)
+    depth = (m * G + math.sqrt((m * G) ** 2 + k * m * u * u)) / k
+    assert abs(qs[:, 6].min() - (-0.1 - depth)) < 1e-4
+    assert tops and abs(tops[0]) < 1e-4
+    s = at_top[0]
+    assert s[0] == 0.0 and np.allclose(s[5:8], [0, 0, -1], atol=1e-12) and s[8] == l_rest
