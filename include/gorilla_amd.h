@@ -247,9 +247,10 @@ int gp_simulate(gp_batch* batch, double final_time, double dt, gp_control_fn con
                 void* user, int32_t integrator, double* qs, double* vs, int64_t max_records,
                 int64_t* n_records);
 
-/* Per-instance fault flags (1 = a non-finite q/v/vdot or a non-positive D was
- * seen since the last clear). No reference counterpart: the reference panics
- * (dynamics.rs:267-274). */
+/* Per-instance fault flags (1 = a non-finite q/v/vdot, a non-positive D or a
+ * spring contact pulling into its half-space was seen since the last clear).
+ * No reference counterpart: the reference panics (dynamics.rs:267-274,
+ * contact.rs:161-163). */
 int gp_batch_faults(gp_batch* batch, uint8_t* faults);
 int gp_batch_clear_faults(gp_batch* batch);
 
