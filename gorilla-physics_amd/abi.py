@@ -149,7 +149,8 @@ SIGNATURES = {
     "gp_batch_set_tau_dev": (C.c_int, [_P, C.c_void_p, _P]),
     "gp_batch_get_vdot_dev": (C.c_int, [_P, C.c_void_p, _P]),
     "gp_batch_set_spring_contact": (C.c_int, [_P, C.c_int64, C.c_int32, _D, C.c_double]),
-    "gp_batch_get_spring_contact": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), _D, _D, _D]),
+    "gp_batch_get_spring_contact": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                              _D, _D, _D]),
     "gp_batch_autotune": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, C.POINTER(gp_batch_opts), _D]),
     "gp_server_create": (C.c_int, [C.POINTER(gp_server_opts), C.POINTER(_P)]),
     "gp_server_destroy": (None, [_P]),

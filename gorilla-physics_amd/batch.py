@@ -184,12 +184,16 @@ class Batch:
         abi.check(self.lib.gp_batch_set_spring_contact(self.handle, int(instance), int(index), _dp(d),
                                                        float(l_rest)), self.lib)
 
-    def get_spring_contact(self, instance, index):
-        """(registered, world point, direction in the body frame, rest length)."""
-        reg = C.c_int32()
+    def get_spring_contact(self, instance, index, with_halfspace=False):
+        """(registered, world point, direction in the body frame, rest length),
+        with the registered half-space's index (-1 if none) after `registered`
+        when with_halfspace."""
+        reg, h = C.c_int32(), C.c_int32()
         p, d, lr = np.zeros(3), np.zeros(3), C.c_double()
-        abi.check(self.lib.gp_batch_get_spring_contact(self.handle, int(instance), int(index), C.byref(reg), _dp(p),
-                                                       _dp(d), C.byref(lr)), self.lib)
+        abi.check(self.lib.gp_batch_get_spring_contact(self.handle, int(instance), int(index), C.byref(reg),
+                                                       C.byref(h), _dp(p), _dp(d), C.byref(lr)), self.lib)
+        if with_halfspace:
+            return bool(reg.value), h.value, p, d, lr.value
         return bool(reg.value), p, d, lr.value
 
     def autotune(self, dt, integrator=abi.GP_SEMI_IMPLICIT_EULER, trial_steps=0):

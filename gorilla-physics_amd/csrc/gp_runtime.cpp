@@ -772,8 +772,8 @@ int gp_batch_set_spring_contact(gp_batch* bt, int64_t inst, int32_t index, const
   return GP_OK;
 }
 
-int gp_batch_get_spring_contact(gp_batch* bt, int64_t inst, int32_t index, int32_t* registered, double* point,
-                                double* direction, double* l_rest) {
+int gp_batch_get_spring_contact(gp_batch* bt, int64_t inst, int32_t index, int32_t* registered, int32_t* halfspace,
+                                double* point, double* direction, double* l_rest) {
   if (!bt) return fail(GP_ERR_INVALID, "null batch");
   if (inst < 0 || inst >= bt->n_inst) return fail(GP_ERR_INVALID, "instance %lld out of range", (long long)inst);
   Bucket& b = bt->buckets[bt->inst_bucket[inst]];
@@ -790,7 +790,9 @@ int gp_batch_get_spring_contact(gp_batch* bt, int64_t inst, int32_t index, int32
     (void)hipMemcpy(&x, b.d_ss + ((size_t)f * b.nsp + s) * b.Bp + l, 8, hipMemcpyDeviceToHost);
     return x;
   };
-  if (registered) *registered = ss(SS_REG) != 0.0 ? 1 : 0;
+  const bool reg = ss(SS_REG) != 0.0;
+  if (registered) *registered = reg ? 1 : 0;
+  if (halfspace) *halfspace = reg ? (int32_t)ss(SS_H) : -1;
   if (point)
     for (int k = 0; k < 3; ++k) point[k] = ss(SS_P + k);
   if (direction) {  // back to the reference's body frame: A d (A^T stored row-major)

@@ -203,7 +203,11 @@ class ContactPoint:
 
 class SpringContact:
     """A stateful spring leg from a body's origin (contact.rs:74-94): direction
-    in the body frame (unit), rest length, stiffness."""
+    in the body frame (unit), rest length, stiffness, and the registration the
+    dynamics keeps (the world contact point and the half-space, None while the
+    leg is in the air). step / simulate / dynamics_continuous send direction
+    and rest length to the device when the caller changed them and read the
+    registration, direction and rest length back afterwards."""
 
     def __init__(self, frame, l_rest, direction, k):
         self.frame = frame
@@ -211,6 +215,8 @@ class SpringContact:
         d = np.asarray(direction, dtype=float)
         self.direction = d / np.linalg.norm(d)
         self.k = float(k)
+        self.registered_contact = None
+        self.registered_halfspace = None
 
     @staticmethod
     def new(frame, l_rest, direction, k):
@@ -477,6 +483,7 @@ class MechanismState:
         self._desc = None
         self._model = None
         self._batch = None
+        self._spring_seen = None
 
     # -- model edits (invalidate the device copy)
     def add_halfspace(self, halfspace):
@@ -500,6 +507,11 @@ class MechanismState:
         self._batch = None
         self._model = None
         self._desc = None
+        self._spring_seen = None
+
+    def spring_contacts(self):
+        """Spring contacts in description order (body by body)."""
+        return [sc for b in self.bodies for sc in b.spring_contacts]
 
     # -- state
     def update(self, q, v):
@@ -540,7 +552,25 @@ class MechanismState:
         if self._batch is None:
             self._model = Model(self.model_desc())
             self._batch = Batch([self._model], [1], opts)
+            self._spring_seen = [(tuple(sc.direction), sc.l_rest) for sc in self.spring_contacts()]
         return self._batch
+
+    def _push_springs(self, b):
+        """Send leg directions / rest lengths the caller changed since the
+        device last reported them (contact.rs:887-888 writes them directly)."""
+        for j, sc in enumerate(self.spring_contacts()):
+            key = (tuple(np.asarray(sc.direction, dtype=float)), float(sc.l_rest))
+            if key != self._spring_seen[j]:
+                b.set_spring_contact(0, j, key[0], key[1])
+                self._spring_seen[j] = key
+
+    def _pull_springs(self, b):
+        for j, sc in enumerate(self.spring_contacts()):
+            reg, h, p, d, lr = b.get_spring_contact(0, j, with_halfspace=True)
+            sc.registered_contact = p if reg else None
+            sc.registered_halfspace = self.halfspaces[h] if reg else None
+            sc.direction, sc.l_rest = d, lr
+            self._spring_seen[j] = (tuple(d), lr)
 
 
 def _tau_flat(state, tau):
@@ -557,7 +587,9 @@ def dynamics_continuous(state, tau):
     """Joint accelerations at the state's (q, v) (dynamics.rs:322)."""
     b = state._device_batch()
     b.set_state(state.flat_q(), state.flat_v())
+    state._push_springs(b)
     vdot = b.dynamics(_tau_flat(state, tau))
+    state._pull_springs(b)
     return unflatten_v(vdot, state.treejoints)
 
 
@@ -566,7 +598,9 @@ def step(state, dt, tau, integrator):
     b = state._device_batch()
     b.set_state(state.flat_q(), state.flat_v())
     b.set_tau(_tau_flat(state, tau))
+    state._push_springs(b)
     b.step(dt, integrator, 1)
+    state._pull_springs(b)
     qf, vf = b.get_state()
     q, v = unflatten_q(qf, state.treejoints), unflatten_v(vf, state.treejoints)
     state.update(q, v)
@@ -582,8 +616,13 @@ def simulate(state, final_time, dt, control_fn, integrator):
 
     def cb(q, v, t):
         state.update(unflatten_q(q, joints), unflatten_v(v, joints))
-        return _tau_flat(state, control_fn(state))
+        state._pull_springs(b)  # a controller may read and move the legs (SLIP_control.rs:120-121)
+        tau = _tau_flat(state, control_fn(state))
+        state._push_springs(b)
+        return tau
 
+    state._push_springs(b)
     qs, vs = b.simulate(final_time, dt, cb, integrator)
+    state._pull_springs(b)
     state.update(unflatten_q(qs[-1], joints), unflatten_v(vs[-1], joints))
     return ([unflatten_q(r, joints) for r in qs], [unflatten_v(r, joints) for r in vs])

@@ -199,13 +199,15 @@ int gp_batch_set_tau(gp_batch* batch, const double* tau);
 
 /* A spring contact's direction (body frame, unit) and rest length for one
  * instance, as the reference's controllers set them between steps
- * (SLIP_control.rs:120-121, contact.rs:887-888); and its registration state:
- * registered (0/1), the registered world point, the current direction and rest
- * length. index counts the model's spring contacts in description order. */
+ * (SLIP_control.rs:120-121, contact.rs:887-888); and its registration state
+ * (contact.rs:78-79): registered (0/1), the registered half-space (its index in
+ * the model's description, -1 when unregistered), the registered world point,
+ * the current direction and rest length. index counts the model's spring
+ * contacts in description order. Any output pointer may be NULL. */
 int gp_batch_set_spring_contact(gp_batch* batch, int64_t instance, int32_t index, const double* direction,
                                 double l_rest);
 int gp_batch_get_spring_contact(gp_batch* batch, int64_t instance, int32_t index, int32_t* registered,
-                                double* point, double* direction, double* l_rest);
+                                int32_t* halfspace, double* point, double* direction, double* l_rest);
 
 /* The same transfers with DEVICE arrays in the same instance-major flattening,
  * asynchronous on `stream` (a hipStream_t; NULL = the batch's stream): the

@@ -178,8 +178,8 @@ def test_SLIP_hopping(gp, oracle):  # contact.rs:836
                 continue
             assert faults[i] == 0, (i, step)
             worst = max(worst, np.abs(qn[7 * i:7 * i + 7] - rq).max(), np.abs(vn[6 * i:6 * i + 6] - rv).max())
-            reg, p, dr, lr = b.get_spring_contact(i, 0)
-            assert reg == bool(ostate[i][0])
+            reg, h, p, dr, lr = b.get_spring_contact(i, 0, with_halfspace=True)
+            assert reg == bool(ostate[i][0]) and h == (0 if reg else -1)
             if reg:
                 assert np.abs(p - ostate[i][1:4]).max() < 1e-10
                 ostate[i][1:4] = p  # the oracle continues from the GPU's state
@@ -219,3 +219,43 @@ def test_spring_contacts_reject_runge_kutta(gp):
             b.step(1e-3, integrator, 1)
         assert e.value.code == gp.abi.GP_ERR_UNSUPPORTED
     b.step(1e-3, 0, 1)
+
+
+def test_SLIP_hopping_through_the_mechanism_mirror(gp, oracle):  # contact.rs:836
+    """The same test written as the reference writes it, against the Python
+    mirror of MechanismState / step: the leg is reset by assigning
+    bodies[0].spring_contacts[0].direction and .l_rest at every apex, and the
+    registration is visible on the SpringContact between steps."""
+    from gorilla_physics_amd import builders
+    print("case reference SLIP_hopping (mechanism mirror)", flush=True)
+    m, r, l_rest, angle, k = 0.54, 0.1, 0.2, math.radians(45.0), 500.0
+    direction = np.array([math.sin(angle), 0.0, -math.cos(angle)])
+    state = builders.build_SLIP(m, r, l_rest, angle, k)
+    state.add_halfspace(gp.HalfSpace.new([0, 0, 1], -0.3))
+    state.update([gp.Pose(gp.UnitQuaternion.identity(), [0.0, 0.0, 0.0])],
+                 [gp.SpatialVector([0.0, 0.0, 0.0], [5.0, 0.0, 0.0])])
+    d = state.model_desc()
+
+    def energy():
+        return sum(oracle.energy(d, state.flat_q(), state.flat_v()))
+
+    e0 = energy()
+    dt = 1.0 / 2000.0
+    v_z_prev, energies, hs, stance = 0.0, [], [], 0
+    for _ in range(int(3.0 / dt)):
+        q, v = gp.step(state, dt, [], gp.Integrator.SemiImplicitEuler)
+        sc = state.bodies[0].spring_contacts[0]
+        if sc.registered_contact is not None:
+            stance += 1
+            assert sc.registered_halfspace is state.halfspaces[0]
+        v_z = v[0].linear[2]
+        if v_z_prev > 0.0 and v_z <= 0.0:
+            energies.append(energy())
+            hs.append(q[0].translation[2])
+            state.bodies[0].spring_contacts[0].direction = direction
+            state.bodies[0].spring_contacts[0].l_rest = l_rest
+        v_z_prev = v_z
+    assert len(energies) > 2 and stance > 0
+    for e in energies:
+        assert abs(e - e0) < 1e-2
+    assert abs(hs[-3] - hs[-2]) < 1e-3 and abs(hs[-2] - hs[-1]) < 1e-3
