@@ -9,11 +9,10 @@ The directory name is not a Python identifier; load it with `load()` below
 (or importlib) under the module name `gorilla_physics_amd`.
 """
 from . import abi  # noqa: F401
-from .mechanism import (GRAVITY, WORLD_FRAME, ContactPoint, FixedJoint, FloatingJoint, HalfSpace,  # noqa: F401
-                        Integrator, Isometry3, JointSpring, MechanismState, Pose, PrismaticJoint, RevoluteJoint,
-                        RigidBody, SpatialInertia, SpatialVector, SpringContact, Transform3D, UnitQuaternion,
-                        dynamics_continuous,
-                        simulate, step)
+from .mechanism import (GRAVITY, WORLD_FRAME, Collider, ContactPoint, FixedJoint, FloatingJoint,  # noqa: F401
+                        HalfSpace, Integrator, Isometry3, JointSpring, MechanismState, Pose, PrismaticJoint,
+                        RevoluteJoint, RigidBody, SpatialInertia, SpatialVector, Sphere, SpringContact, Transform3D,
+                        UnitQuaternion, dynamics_continuous, simulate, step)
 
 
 def load():

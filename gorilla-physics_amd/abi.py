@@ -27,6 +27,8 @@ GP_JOINT_FLOATING = 3
 GP_SEMI_IMPLICIT_EULER = 0
 GP_RUNGE_KUTTA_2 = 1
 GP_RUNGE_KUTTA_4 = 2
+GP_VELOCITY_STEPPING = 3
+GP_CCD_VELOCITY_STEPPING = 4
 
 GP_PREC_F64 = 0
 GP_PREC_MIXED = 1
@@ -69,6 +71,10 @@ class gp_spring_contact_desc(C.Structure):
                 ("k", C.c_double)]
 
 
+class gp_sphere_desc(C.Structure):
+    _fields_ = [("body", C.c_int32), ("enabled", C.c_int32), ("center", C.c_double * 3), ("radius", C.c_double)]
+
+
 class gp_model_desc(C.Structure):
     _fields_ = [
         ("n_bodies", C.c_int32),
@@ -82,6 +88,9 @@ class gp_model_desc(C.Structure):
         ("n_spring_contacts", C.c_int32),
         ("_pad2", C.c_int32),
         ("spring_contacts", C.POINTER(gp_spring_contact_desc)),
+        ("n_spheres", C.c_int32),
+        ("_pad3", C.c_int32),
+        ("spheres", C.POINTER(gp_sphere_desc)),
     ]
 
 

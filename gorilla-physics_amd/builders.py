@@ -47,6 +47,21 @@ def build_cube(mass, length):  # helpers.rs:151
     return state
 
 
+def build_sphere(mass, radius):  # helpers.rs:812
+    """A floating solid sphere with its sphere collider."""
+    body = RigidBody.new_sphere(mass, radius, "sphere")
+    return MechanismState([FloatingJoint(Transform3D.identity("sphere", WORLD_FRAME))], [body])
+
+
+def build_n_spheres(mass, radius, n):  # helpers.rs:823
+    bodies, joints = [], []
+    for i in range(1, n + 1):
+        frame = f"sphere {i}"
+        bodies.append(RigidBody.new_sphere(mass, radius, frame))
+        joints.append(FloatingJoint(Transform3D.identity(frame, WORLD_FRAME)))
+    return MechanismState(joints, bodies)
+
+
 def build_SLIP(m, r, l_rest, angle, k_spring):  # helpers.rs:308
     """Spring-loaded inverted pendulum: a floating solid sphere (moment
     2/5 m r^2) with one spring leg at `angle` from straight down, forward."""

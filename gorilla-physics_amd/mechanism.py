@@ -244,11 +244,38 @@ class HalfSpace:
         return float(np.dot(np.asarray(point) - self.point, self.normal)) <= 1e-8
 
 
+class Sphere:
+    """Sphere collider (collision/sphere.rs:6-20): centre at base_translation
+    in the body frame."""
+
+    def __init__(self, radius):
+        self.base_translation = np.zeros(3)
+        self.radius = float(radius)
+
+    @staticmethod
+    def new(radius):
+        return Sphere(radius)
+
+
+class Collider:
+    """A body's collision geometry (rigid_body.rs:11-37); only spheres reach
+    the device (the velocity-stepping integrators use them)."""
+
+    def __init__(self, geometry):
+        self.geometry = geometry
+        self.enabled = True
+
+    @staticmethod
+    def new_sphere(sphere):
+        return Collider(sphere)
+
+
 class RigidBody:
     def __init__(self, inertia):
         self.inertia = inertia
         self.contact_points = []
         self.spring_contacts = []
+        self.collider = None
 
     @staticmethod
     def new(inertia):
@@ -257,7 +284,9 @@ class RigidBody:
     @staticmethod
     def new_sphere(m, r, frame):  # rigid_body.rs:114
         mx = 2.0 / 5.0 * m * r * r
-        return RigidBody(SpatialInertia(frame, np.diag([mx, mx, mx]), np.zeros(3), m))
+        body = RigidBody(SpatialInertia(frame, np.diag([mx, mx, mx]), np.zeros(3), m))
+        body.collider = Collider.new_sphere(Sphere(r))
+        return body
 
     @staticmethod
     def new_cube(m, l, frame):  # rigid_body.rs:147
@@ -282,6 +311,9 @@ class RigidBody:
         if self.inertia.frame != sc.frame:
             raise ValueError(f"Spring contact point frame {sc.frame} does not match body frame {self.inertia.frame}")
         self.spring_contacts.append(sc)
+
+    def add_sphere_collider(self, sphere):  # rigid_body.rs:201
+        self.collider = Collider.new_sphere(sphere)
 
 
 # --------------------------------------------------------------------- joints
@@ -332,21 +364,24 @@ class Integrator:  # integrators.rs:17
     SemiImplicitEuler = abi.GP_SEMI_IMPLICIT_EULER
     RungeKutta2 = abi.GP_RUNGE_KUTTA_2
     RungeKutta4 = abi.GP_RUNGE_KUTTA_4
+    VelocityStepping = abi.GP_VELOCITY_STEPPING
+    CCDVelocityStepping = abi.GP_CCD_VELOCITY_STEPPING
 
 
 # --------------------------------------------------------------------- model description
 class ModelDesc:
     """Owner of the POD arrays behind one gp_model_desc."""
 
-    def __init__(self, bodies, contacts, halfspaces, gravity=GRAVITY, spring_contacts=()):
+    def __init__(self, bodies, contacts, halfspaces, gravity=GRAVITY, spring_contacts=(), spheres=()):
         self.n_bodies = len(bodies)
         self._bodies = (abi.gp_body_desc * max(1, len(bodies)))(*bodies)
         self._contacts = (abi.gp_contact_desc * max(1, len(contacts)))(*contacts)
         self._halfspaces = (abi.gp_halfspace_desc * max(1, len(halfspaces)))(*halfspaces)
         self._springs = (abi.gp_spring_contact_desc * max(1, len(spring_contacts)))(*spring_contacts)
+        self._spheres = (abi.gp_sphere_desc * max(1, len(spheres)))(*spheres)
         self.desc = abi.gp_model_desc(len(bodies), len(contacts), len(halfspaces), 0,
                                       self._bodies, self._contacts, self._halfspaces, gravity,
-                                      len(spring_contacts), 0, self._springs)
+                                      len(spring_contacts), 0, self._springs, len(spheres), 0, self._spheres)
         self.nq = sum(7 if b.jtype == abi.GP_JOINT_FLOATING else (0 if b.jtype == abi.GP_JOINT_FIXED else 1)
                       for b in bodies)
         self.nv = sum(6 if b.jtype == abi.GP_JOINT_FLOATING else (0 if b.jtype == abi.GP_JOINT_FIXED else 1)
@@ -385,6 +420,15 @@ def spring_contact_desc(body, sc):
     d.body = body
     d.direction[:] = [float(x) for x in sc.direction]
     d.l_rest, d.k = sc.l_rest, sc.k
+    return d
+
+
+def sphere_desc(body, collider):
+    d = abi.gp_sphere_desc()
+    d.body = body
+    d.enabled = 1 if collider.enabled else 0
+    d.center[:] = [float(x) for x in collider.geometry.base_translation]
+    d.radius = collider.geometry.radius
     return d
 
 
@@ -526,6 +570,22 @@ class MechanismState:
     def set_joint_v(self, jointid, v):
         self.v[jointid - 1] = v
 
+    def poses(self):
+        """World pose of every body at the current q (mechanism.rs:403-418)."""
+        world, out = [], []
+        for i, j in enumerate(self.treejoints):
+            iso, q = j.transform.iso, self.q[i]
+            if j.jtype == abi.GP_JOINT_REVOLUTE:
+                iso = iso * Isometry3(UnitQuaternion.from_axis_angle(j.axis, q))
+            elif j.jtype == abi.GP_JOINT_PRISMATIC:
+                iso = iso * Isometry3(None, j.axis * q)
+            elif j.jtype == abi.GP_JOINT_FLOATING:
+                iso = iso * Isometry3(q.rotation, q.translation)
+            b2r = iso if self.parents[i] == 0 else world[self.parents[i] - 1] * iso
+            world.append(b2r)
+            out.append(Pose(b2r.rotation, b2r.translation))
+        return out
+
     def flat_q(self):
         return np.array(flatten_q(self.q), dtype=float)
 
@@ -535,7 +595,7 @@ class MechanismState:
     # -- model description in the C ABI's terms
     def model_desc(self):
         if self._desc is None:
-            bodies, contacts, springs = [], [], []
+            bodies, contacts, springs, spheres = [], [], [], []
             for i, (j, b) in enumerate(zip(self.treejoints, self.bodies)):
                 spring = getattr(j, "spring", None)
                 bodies.append(body_desc(self.parents[i] - 1, j.jtype, j.axis, j.transform.iso, b.inertia, spring))
@@ -543,8 +603,10 @@ class MechanismState:
                     contacts.append(contact_desc(i, cp.location, cp.k))
                 for sc in b.spring_contacts:
                     springs.append(spring_contact_desc(i, sc))
+                if b.collider is not None and isinstance(b.collider.geometry, Sphere):
+                    spheres.append(sphere_desc(i, b.collider))
             hs = [halfspace_desc(h) for h in self.halfspaces]
-            self._desc = ModelDesc(bodies, contacts, hs, self.gravity, springs)
+            self._desc = ModelDesc(bodies, contacts, hs, self.gravity, springs, spheres)
         return self._desc
 
     def _device_batch(self, opts=None):

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GP_ABI_VERSION 2
+#define GP_ABI_VERSION 3
 
 /* ---- status codes (replace the reference's panic!/expect sites) ---------- */
 enum {
@@ -59,7 +59,16 @@ enum {
 enum {
   GP_SEMI_IMPLICIT_EULER = 0,
   GP_RUNGE_KUTTA_2 = 1,
-  GP_RUNGE_KUTTA_4 = 2
+  GP_RUNGE_KUTTA_4 = 2,
+  /* velocity stepping (integrators.rs:42-49, dynamics.rs:366-742): free velocity
+   * without contact forces, then the contact impulses of a second-order-cone
+   * program (friction cone mu = 0.95, dynamics.rs:674) over the touching
+   * contact points and sphere colliders; q advanced with the new velocity */
+  GP_VELOCITY_STEPPING = 3,
+  /* the same with continuous collision detection (integrators.rs:51-175): the
+   * step is cut at the earliest time a separated contact point or sphere would
+   * reach a half-space, and the remainder stepped again */
+  GP_CCD_VELOCITY_STEPPING = 4
 };
 
 /* ---- arithmetic of the batched step --------------------------------------- */
@@ -126,6 +135,18 @@ typedef struct gp_spring_contact_desc {
   double k;
 } gp_spring_contact_desc;
 
+/* Sphere collider of a body (collision/sphere.rs:6-39, rigid_body.rs:11-37):
+ * centre at `center` in the body frame (Sphere::base_translation), radius.
+ * Touches a half-space when the centre is at most `radius` above it; used by
+ * the velocity-stepping integrators only, as in the reference. A body has at
+ * most one collider; enabled = 0 switches it off (Collider::enabled). */
+typedef struct gp_sphere_desc {
+  int32_t body;
+  int32_t enabled;
+  double center[3];
+  double radius;
+} gp_sphere_desc;
+
 typedef struct gp_model_desc {
   int32_t n_bodies;
   int32_t n_contacts;
@@ -138,6 +159,9 @@ typedef struct gp_model_desc {
   int32_t n_spring_contacts;  /* zero-initialised descriptions have none */
   int32_t _pad2;
   const gp_spring_contact_desc* spring_contacts;
+  int32_t n_spheres;          /* sphere colliders, at most one per body */
+  int32_t _pad3;
+  const gp_sphere_desc* spheres;
 } gp_model_desc;
 
 /* Options of a batch. Zero-initialised = defaults. */

@@ -110,6 +110,27 @@ int oracle_step_spring(const gp_model_desc* d, const double* q, const double* v,
   return st;
 }
 
+// velocity_stepping / ccd_velocity_stepping (integrators.rs:42-175): one step.
+// Returns 0, 1 (singular mass matrix), 3 (no finite impulse) or 4 (the
+// continuous-collision loop did not finish); *rounds = the loop's rounds.
+int oracle_vs_step(const gp_model_desc* d, int ccd, const double* q, const double* v, const double* tau, double dt,
+                   double* q_out, double* v_out, int* rounds) {
+  Model<double> M = load_model<double>(*d);
+  std::vector<double> tt(M.nv, 0.0);
+  if (tau) for (int j = 0; j < M.nv; ++j) tt[j] = tau[j];
+  return vs_step(M, ccd != 0, q, v, tt.data(), dt, q_out, v_out, rounds);
+}
+
+// The contact impulse program alone (dynamics.rs:668-704): G (3nc x 3nc,
+// row-major), g -> x; *iters = interior-point iterations.
+int oracle_cone_program(int nc, const double* G, const double* g, double* x, int* iters) {
+  const int m = 3 * nc;
+  std::vector<double> Gv(G, G + size_t(m) * m), gv(g, g + m), xv;
+  int st = cone_program(Gv, gv, nc, xv, iters);
+  for (int k = 0; k < m; ++k) x[k] = xv[k];
+  return st;
+}
+
 // mass matrix (mechanism.rs:637) and bias c(q,v) - tau_contact (dynamics.rs:233)
 int oracle_mass_matrix(const gp_model_desc* d, const double* q, const double* v, double* Mout, double* cout) {
   Model<double> M = load_model<double>(*d);

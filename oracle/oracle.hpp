@@ -20,10 +20,13 @@
 //     whose z axis is each 1-DOF joint's axis — the formulation the GPU kernels
 //     use. Exact in real arithmetic, so it agrees with (1) to rounding.
 //
-// Integrators: semi-implicit Euler (integrators.rs:276-319, util.rs:83-102) and
-// RK2 / RK4 (integrators.rs:177-271).
+// Integrators: semi-implicit Euler (integrators.rs:276-319, util.rs:83-102),
+// RK2 / RK4 (integrators.rs:177-271), and velocity stepping with and without
+// continuous collision detection (integrators.rs:42-175, dynamics.rs:366-742):
+// contact impulses from a second-order-cone program (cone_program).
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -172,6 +175,14 @@ struct HalfSpace {
   V3<T> point, normal;
   T alpha, mu;
 };
+// Sphere collider (collision/sphere.rs:6-39): centre in the body frame
+// (base_translation), radius; Collider::enabled (rigid_body.rs:11-37)
+template <class T>
+struct SphereC {
+  bool present = false, enabled = false;
+  V3<T> c;
+  T r = T(0);
+};
 
 template <class T>
 struct Model {
@@ -185,6 +196,7 @@ struct Model {
   std::vector<std::vector<Contact<T>>> contacts;  // per body
   std::vector<std::vector<SpringC<T>>> springs;   // per body, description order within a body
   std::vector<HalfSpace<T>> halfspaces;
+  std::vector<SphereC<T>> spheres;                // per body
   T g = T(9.81);
   std::vector<int> qoff, voff;  // per body
   int nq = 0, nv = 0;
@@ -201,6 +213,7 @@ Model<T> load_model(const gp_model_desc& d) {
   M.parent.resize(M.n); M.jtype.resize(M.n); M.axis.resize(M.n); M.tree.resize(M.n); M.I.resize(M.n);
   M.spring_k.resize(M.n); M.spring_l.resize(M.n); M.contacts.resize(M.n); M.qoff.resize(M.n); M.voff.resize(M.n);
   M.springs.resize(M.n);
+  M.spheres.resize(M.n);
   for (int i = 0; i < M.n; ++i) {
     const gp_body_desc& b = d.bodies[i];
     M.parent[i] = b.parent;
@@ -229,6 +242,14 @@ Model<T> load_model(const gp_model_desc& d) {
     const gp_spring_contact_desc& sd = d.spring_contacts[c];
     M.springs[sd.body].push_back({V3<T>(T(sd.direction[0]), T(sd.direction[1]), T(sd.direction[2])), T(sd.l_rest),
                                   T(sd.k)});
+  }
+  for (int s = 0; s < d.n_spheres; ++s) {
+    const gp_sphere_desc& sd = d.spheres[s];
+    SphereC<T>& S = M.spheres[sd.body];
+    S.present = true;
+    S.enabled = sd.enabled != 0;
+    S.c = V3<T>(T(sd.center[0]), T(sd.center[1]), T(sd.center[2]));
+    S.r = T(sd.radius);
   }
   for (int h = 0; h < d.n_halfspaces; ++h) {
     const gp_halfspace_desc& hd = d.halfspaces[h];
@@ -359,7 +380,7 @@ template <class T> T springs_k(const Model<T>& M, int body, size_t s) { return M
 // half-space (contact.rs:161-163).
 template <class T>
 int crba_dynamics(const Model<T>& M, const T* q, const T* v, const T* tau, T* vdot,
-                  T* Mout = nullptr, T* cout = nullptr, SpringSt<T>* ss = nullptr) {
+                  T* Mout = nullptr, T* cout = nullptr, SpringSt<T>* ss = nullptr, bool contact_wrenches = true) {
   const int n = M.n, nv = M.nv;
   std::vector<SpringSt<T>> fresh;  // no caller state: the springs as described, unregistered
   if (!ss) {
@@ -431,9 +452,10 @@ int crba_dynamics(const Model<T>& M, const T* q, const T* v, const T* tau, T* vd
   for (int r = 0; r < nv; ++r)
     for (int c = r + 1; c < nv; ++c) H[size_t(r) * nv + c] = H[size_t(c) * nv + r];
 
-  // ---- contact wrenches (contact.rs:97-129), world frame about world origin
+  // ---- contact wrenches (contact.rs:97-129), world frame about world origin;
+  // none on the velocity-stepping path (dynamics.rs:377-383)
   std::vector<SV<T>> wc(n);
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < n && contact_wrenches; ++i) {
     M3<T> R = qmat(b2r[i].r);
     for (const Contact<T>& cp : M.contacts[i]) {
       V3<T> pw = R * cp.p + b2r[i].t;              // ContactPoint::transform
@@ -626,6 +648,468 @@ int step_integrate(const Model<T>& M, int integrator, const T* q, const T* v, co
   for (int k = 0; k < nv; ++k) af[k] = (a1[k] + a2[k] * T(2) + a3[k] * T(2) + a4[k]) / T(6);
   euler(af.data(), dt, q_out, v_out);
   return 0;
+}
+
+// ---------------------------------------------------------------- velocity stepping
+// Partial-pivot LU (nalgebra 0.33.2 `lu()`: the pivot is the largest |a| of
+// the column; the row swaps are replayed on every right-hand side).
+template <class T>
+struct LUF {
+  int n = 0;
+  std::vector<T> a;      // L below the diagonal (unit diagonal), U on and above
+  std::vector<int> swp;  // row swapped with row k at step k
+  bool ok = true;
+};
+template <class T> LUF<T> lu_factor(const std::vector<T>& A, int n) {
+  LUF<T> f;
+  f.n = n;
+  f.a = A;
+  f.swp.assign(n, 0);
+  for (int col = 0; col < n; ++col) {
+    int p = col;
+    T best = std::fabs(f.a[size_t(col) * n + col]);
+    for (int r = col + 1; r < n; ++r) {
+      T v = std::fabs(f.a[size_t(r) * n + col]);
+      if (v > best) { best = v; p = r; }
+    }
+    f.swp[col] = p;
+    if (best == T(0)) { f.ok = false; return f; }
+    if (p != col)
+      for (int k = 0; k < n; ++k) std::swap(f.a[size_t(p) * n + k], f.a[size_t(col) * n + k]);
+    const T d = f.a[size_t(col) * n + col];
+    for (int r = col + 1; r < n; ++r) {
+      T m = f.a[size_t(r) * n + col] / d;
+      f.a[size_t(r) * n + col] = m;
+      for (int k = col + 1; k < n; ++k) f.a[size_t(r) * n + k] = f.a[size_t(r) * n + k] - m * f.a[size_t(col) * n + k];
+    }
+  }
+  return f;
+}
+template <class T> void lu_solve(const LUF<T>& f, const T* b, T* x) {
+  const int n = f.n;
+  std::vector<T> y(b, b + n);
+  for (int k = 0; k < n; ++k) std::swap(y[k], y[f.swp[k]]);
+  for (int r = 0; r < n; ++r)
+    for (int k = 0; k < r; ++k) y[r] = y[r] - f.a[size_t(r) * n + k] * y[k];
+  for (int r = n - 1; r >= 0; --r) {
+    T s = y[r];
+    for (int k = r + 1; k < n; ++k) s = s - f.a[size_t(r) * n + k] * x[k];
+    x[r] = s / f.a[size_t(r) * n + r];
+  }
+}
+
+// The contact impulse program of dynamics.rs:668-704 (Clarabel 0.11.1 with
+// default settings there): minimise 1/2 x^T G x + g^T x with
+// (mu x_3c, x_3c+1, x_3c+2) in the 3-dimensional second-order cone for every
+// contact c — Clarabel's A = -diag(mu, 1, 1), b = 0, SecondOrderConeT(3),
+// mu = 0.95 (dynamics.rs:674). Clarabel is a Rust crate absent here; it is
+// restated as the method family it implements: a primal-dual interior-point
+// method with Nesterov-Todd scaling and Mehrotra's predictor-corrector
+// (L. Vandenberghe, "The CVXOPT linear and quadratic cone program solvers",
+// 2010, sections 2-4), without Clarabel's homogeneous embedding, run to a
+// relative gap of 1e-16 and residual of 1e-14 (Clarabel stops at 1e-8), or,
+// once rounding stalls the progress for 3 iterations, to the iterate with the
+// smallest of the larger of the two. G is positive
+// semidefinite and the optimum in x may be a set; the velocities it produces
+// are unique. Returns 0, or 1 if the iterations produced no finite point.
+constexpr double kConeMu = 0.95;
+constexpr int kConeMaxIter = 50;
+constexpr int kConeStall = 3;  // iterations without a better iterate
+constexpr double kConeGapTol = 1e-16;
+constexpr double kConeResTol = 1e-14;
+
+template <class T> struct Soc3 {
+  T a[3];
+};
+template <class T> T soc_det(const T* u) { return u[0] * u[0] - u[1] * u[1] - u[2] * u[2]; }
+// Nesterov-Todd scaling of one cone: W z = W^-1 s (Vandenberghe 2010, 4.3):
+// W = eta [w0 w1^T; w1 I + w1 w1^T / (1 + w0)], w = (s/|s| + R z/|z|) / (2 gamma)
+template <class T> void nt_scaling(const T* s, const T* z, T W[3][3], T Wi[3][3]) {
+  using std::sqrt;
+  T ds = sqrt(soc_det(s)), dz = sqrt(soc_det(z));
+  T sb[3] = {s[0] / ds, s[1] / ds, s[2] / ds}, zb[3] = {z[0] / dz, z[1] / dz, z[2] / dz};
+  T gam = sqrt((T(1) + sb[0] * zb[0] + sb[1] * zb[1] + sb[2] * zb[2]) / T(2));
+  T w[3] = {(sb[0] + zb[0]) / (T(2) * gam), (sb[1] - zb[1]) / (T(2) * gam), (sb[2] - zb[2]) / (T(2) * gam)};
+  T eta = sqrt(ds / dz);
+  T f = T(1) / (T(1) + w[0]);
+  T B[3][3] = {{w[0], w[1], w[2]}, {w[1], T(1) + f * w[1] * w[1], f * w[1] * w[2]}, {w[2], f * w[1] * w[2], T(1) + f * w[2] * w[2]}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      W[i][j] = eta * B[i][j];
+      T sg = ((i == 0) != (j == 0)) ? T(-1) : T(1);
+      Wi[i][j] = sg * B[i][j] / eta;
+    }
+}
+// u o v and the y solving u o y = w in the Jordan algebra of the cone
+template <class T> void soc_prod(const T* u, const T* v, T* out) {
+  out[0] = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+  out[1] = u[0] * v[1] + v[0] * u[1];
+  out[2] = u[0] * v[2] + v[0] * u[2];
+}
+template <class T> void soc_div(const T* u, const T* w, T* y) {
+  y[0] = (u[0] * w[0] - u[1] * w[1] - u[2] * w[2]) / soc_det(u);
+  y[1] = (w[1] - y[0] * u[1]) / u[0];
+  y[2] = (w[2] - y[0] * u[2]) / u[0];
+}
+// largest step a with u + a du still in the cone (first positive root of det)
+template <class T> T soc_max_step(const T* u, const T* du) {
+  using std::sqrt;
+  T a = soc_det(du);
+  T b = u[0] * du[0] - u[1] * du[1] - u[2] * du[2];
+  T c = soc_det(u);
+  T best = T(1e300);
+  if (a == T(0)) {
+    if (b < T(0)) best = -c / (T(2) * b);
+    return best;
+  }
+  T disc = b * b - a * c;
+  if (disc < T(0)) return best;
+  T q = -(b + (b < T(0) ? -sqrt(disc) : sqrt(disc)));
+  T r1 = q / a, r2 = q != T(0) ? c / q : T(1e300);
+  if (r1 > T(0) && r1 < best) best = r1;
+  if (r2 > T(0) && r2 < best) best = r2;
+  return best;
+}
+
+template <class T>
+int cone_program(const std::vector<T>& G, const std::vector<T>& g, int nc, std::vector<T>& x, int* iters = nullptr) {
+  using std::fabs; using std::sqrt;
+  const int m = 3 * nc;
+  const T mu = T(kConeMu);
+  std::vector<T> d(m), z(m), s(m), rd(m), lt(m), xbest, dx(m), dz(m), ds(m), xi(m), rhs(m), L(size_t(m) * m);
+  std::vector<std::array<std::array<T, 3>, 3>> W(nc), Wi(nc);
+  x.assign(m, T(0));
+  for (int c = 0; c < nc; ++c) {
+    d[3 * c] = mu; d[3 * c + 1] = T(1); d[3 * c + 2] = T(1);
+    x[3 * c] = T(1) / mu;
+    z[3 * c] = T(1); z[3 * c + 1] = T(0); z[3 * c + 2] = T(0);
+  }
+  T gnorm = T(0);
+  for (int k = 0; k < m; ++k) gnorm = std::fmax(gnorm, fabs(g[k]));
+  T best = T(1e300);
+  xbest = x;
+  int it = 0, since_best = 0;
+  auto solve = [&](const std::vector<T>& xi_, std::vector<T>& dx_, std::vector<T>& dz_) {
+    for (int c = 0; c < nc; ++c)
+      for (int i = 0; i < 3; ++i) {
+        T t = T(0);
+        for (int j = 0; j < 3; ++j) t = t + Wi[c][i][j] * xi_[3 * c + j];
+        rhs[3 * c + i] = -rd[3 * c + i] + d[3 * c + i] * t;
+      }
+    for (int r = 0; r < m; ++r) {  // L y = rhs
+      T t = rhs[r];
+      for (int k = 0; k < r; ++k) t = t - L[size_t(r) * m + k] * dx_[k];
+      dx_[r] = t / L[size_t(r) * m + r];
+    }
+    for (int r = m - 1; r >= 0; --r) {  // L^T dx = y
+      T t = dx_[r];
+      for (int k = r + 1; k < m; ++k) t = t - L[size_t(k) * m + r] * dx_[k];
+      dx_[r] = t / L[size_t(r) * m + r];
+    }
+    for (int c = 0; c < nc; ++c) {
+      T a[3], b[3];
+      for (int i = 0; i < 3; ++i) {
+        a[i] = T(0);
+        for (int j = 0; j < 3; ++j) a[i] = a[i] + Wi[c][i][j] * d[3 * c + j] * dx_[3 * c + j];
+      }
+      for (int i = 0; i < 3; ++i) {
+        b[i] = T(0);
+        for (int j = 0; j < 3; ++j) b[i] = b[i] + Wi[c][i][j] * (xi_[3 * c + j] - a[j]);
+      }
+      for (int i = 0; i < 3; ++i) dz_[3 * c + i] = b[i];
+    }
+  };
+  for (it = 0; it < kConeMaxIter; ++it) {
+    T gap = T(0), rmax = T(0), xmax = T(0);
+    for (int k = 0; k < m; ++k) s[k] = d[k] * x[k];
+    for (int r = 0; r < m; ++r) {
+      T t = g[r] - d[r] * z[r];
+      for (int k = 0; k < m; ++k) t = t + G[size_t(r) * m + k] * x[k];
+      rd[r] = t;
+      rmax = std::fmax(rmax, fabs(t));
+      gap = gap + s[r] * z[r];
+      xmax = std::fmax(xmax, fabs(x[r]));
+    }
+    bool finite = std::isfinite(gap) && std::isfinite(rmax);
+    if (!finite) break;
+    // relative gap (impulse x velocity scale) and residual (velocity scale)
+    const T gap_rel = gap / ((T(1) + gnorm) * (T(1) + xmax));
+    const T res_rel = rmax / (T(1) + gnorm);
+    const T merit = std::fmax(gap_rel, res_rel);
+    if (merit < best) { best = merit; xbest = x; since_best = 0; }
+    if (gap_rel <= T(kConeGapTol) && res_rel <= T(kConeResTol)) break;
+    if (++since_best > kConeStall) break;  // rounding has stopped the progress
+    const T mug = gap / T(nc);
+    for (int c = 0; c < nc; ++c) {
+      T Wc[3][3], Wic[3][3];
+      nt_scaling(&s[3 * c], &z[3 * c], Wc, Wic);
+      for (int i = 0; i < 3; ++i) {
+        lt[3 * c + i] = Wc[i][0] * z[3 * c] + Wc[i][1] * z[3 * c + 1] + Wc[i][2] * z[3 * c + 2];
+        for (int j = 0; j < 3; ++j) { W[c][i][j] = Wc[i][j]; Wi[c][i][j] = Wic[i][j]; }
+      }
+    }
+    // K = G + D W^-2 D (block diagonal), Cholesky K = L L^T
+    for (size_t k = 0; k < L.size(); ++k) L[k] = G[k];
+    for (int c = 0; c < nc; ++c)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          T t = T(0);
+          for (int k = 0; k < 3; ++k) t = t + Wi[c][i][k] * Wi[c][k][j];
+          L[size_t(3 * c + i) * m + 3 * c + j] = L[size_t(3 * c + i) * m + 3 * c + j] + d[3 * c + i] * t * d[3 * c + j];
+        }
+    bool pd = true;
+    for (int j = 0; j < m && pd; ++j) {
+      T t = L[size_t(j) * m + j];
+      for (int k = 0; k < j; ++k) t = t - L[size_t(j) * m + k] * L[size_t(j) * m + k];
+      if (!(t > T(0))) { pd = false; break; }
+      T ljj = sqrt(t);
+      L[size_t(j) * m + j] = ljj;
+      for (int r = j + 1; r < m; ++r) {
+        T u = L[size_t(r) * m + j];
+        for (int k = 0; k < j; ++k) u = u - L[size_t(r) * m + k] * L[size_t(j) * m + k];
+        L[size_t(r) * m + j] = u / ljj;
+      }
+    }
+    if (!pd) break;
+    // predictor
+    for (int k = 0; k < m; ++k) xi[k] = -lt[k];
+    solve(xi, dx, dz);
+    T alpha = T(1);
+    for (int k = 0; k < m; ++k) ds[k] = d[k] * dx[k];
+    for (int c = 0; c < nc; ++c)
+      alpha = std::fmin(alpha, std::fmin(soc_max_step(&s[3 * c], &ds[3 * c]), soc_max_step(&z[3 * c], &dz[3 * c])));
+    T sz = T(0);
+    for (int k = 0; k < m; ++k) sz = sz + (s[k] + alpha * ds[k]) * (z[k] + alpha * dz[k]);
+    T ratio = sz / gap;
+    T sigma = ratio * ratio * ratio;
+    // corrector: xi = lt \ (sigma mu e - lt o lt - (W^-1 ds) o (W dz))
+    for (int c = 0; c < nc; ++c) {
+      T a[3], b[3], ll[3], cc[3], w[3];
+      for (int i = 0; i < 3; ++i) {
+        a[i] = Wi[c][i][0] * ds[3 * c] + Wi[c][i][1] * ds[3 * c + 1] + Wi[c][i][2] * ds[3 * c + 2];
+        b[i] = W[c][i][0] * dz[3 * c] + W[c][i][1] * dz[3 * c + 1] + W[c][i][2] * dz[3 * c + 2];
+      }
+      soc_prod(&lt[3 * c], &lt[3 * c], ll);
+      soc_prod(a, b, cc);
+      w[0] = sigma * mug - ll[0] - cc[0];
+      w[1] = -ll[1] - cc[1];
+      w[2] = -ll[2] - cc[2];
+      soc_div(&lt[3 * c], w, &xi[3 * c]);
+    }
+    solve(xi, dx, dz);
+    for (int k = 0; k < m; ++k) ds[k] = d[k] * dx[k];
+    alpha = T(1);
+    for (int c = 0; c < nc; ++c)
+      alpha = std::fmin(alpha, T(0.99) * std::fmin(soc_max_step(&s[3 * c], &ds[3 * c]), soc_max_step(&z[3 * c], &dz[3 * c])));
+    for (int k = 0; k < m; ++k) { x[k] = x[k] + alpha * dx[k]; z[k] = z[k] + alpha * dz[k]; }
+  }
+  if (iters) *iters = it;
+  x = xbest;
+  for (int k = 0; k < m; ++k)
+    if (!std::isfinite(x[k])) return 1;
+  return best < T(1e300) ? 0 : 1;
+}
+
+// One contact of the velocity-stepping path: normal, world point, body
+// (collision_detection, dynamics.rs:546-606: point contacts inside a
+// half-space, then sphere colliders touching one)
+template <class T>
+struct VsContact {
+  V3<T> n, p;
+  int body;
+};
+
+template <class T> std::vector<Iso<T>> bodies_to_root(const Model<T>& M, const T* q) {
+  std::vector<Iso<T>> b2r(M.n);
+  for (int i = 0; i < M.n; ++i) {
+    Iso<T> XJ = joint_transform(M, i, q);
+    b2r[i] = M.parent[i] < 0 ? XJ : imul(b2r[M.parent[i]], XJ);
+  }
+  return b2r;
+}
+
+// dynamics_discrete (dynamics.rs:366-424): free velocity without contact
+// forces, then the impulses of the cone program over the touching contacts
+// (solve_constraint_and_contact, dynamics.rs:647-742). Returns 0, 1 on a
+// singular mass matrix, 3 when the cone program gives no finite impulse
+// (the reference panics on NaN, dynamics.rs:731-733).
+template <class T>
+int vs_dynamics(const Model<T>& M, const T* q, const T* v, const T* tau, T dt, T* v_next,
+                std::vector<VsContact<T>>& contacts, int* cone_iters = nullptr) {
+  using std::sqrt;
+  const int n = M.n, nv = M.nv;
+  std::vector<T> Mm(size_t(nv) * nv), c(nv), scratch(nv), zero(nv, T(0));
+  crba_dynamics(M, q, v, zero.data(), scratch.data(), Mm.data(), c.data(), (SpringSt<T>*)nullptr, false);
+  std::vector<T> rhs(nv), vfree(nv);
+  for (int k = 0; k < nv; ++k) rhs[k] = tau[k];
+  for (int i = 0; i < n; ++i)  // add_prismatic_joint_spring_force (dynamics.rs:298-315)
+    if (M.jtype[i] == GP_JOINT_PRISMATIC && !(M.spring_k[i] == T(0)))
+      rhs[M.voff[i]] = rhs[M.voff[i]] + (-M.spring_k[i] * (q[M.qoff[i]] - M.spring_l[i]));
+  for (int k = 0; k < nv; ++k) rhs[k] = rhs[k] * dt - c[k] * dt;  // dynamics_solve(M, c dt, tau dt)
+  LUF<T> lu = lu_factor(Mm, nv);
+  if (nv > 0 && !lu.ok) return 1;
+  if (nv > 0) lu_solve(lu, rhs.data(), vfree.data());
+  for (int k = 0; k < nv; ++k) vfree[k] = v[k] + vfree[k];
+  // world motion subspaces (build_jacobian_blocks, dynamics.rs:432-471)
+  std::vector<Iso<T>> b2r = bodies_to_root(M, q);
+  std::vector<std::array<SV<T>, 6>> Sw(n);
+  std::vector<int> dof(n);
+  for (int i = 0; i < n; ++i) {
+    SV<T> S[6];
+    dof[i] = motion_subspace(M, i, S);
+    for (int k = 0; k < dof[i]; ++k) Sw[i][k] = jac_col_transform(S[k], b2r[i]);
+  }
+  // collision detection
+  contacts.clear();
+  for (int i = 0; i < n; ++i) {
+    M3<T> R = qmat(b2r[i].r);
+    for (const Contact<T>& cp : M.contacts[i]) {
+      V3<T> pw = R * cp.p + b2r[i].t;
+      for (const HalfSpace<T>& h : M.halfspaces)
+        if (dot(pw - h.point, h.normal) <= T(kInsideMargin)) contacts.push_back({h.normal, pw, i});
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    const SphereC<T>& S = M.spheres[i];
+    if (!S.present || !S.enabled) continue;
+    V3<T> center = b2r[i].t + qrot(b2r[i].r, S.c);  // Sphere::center
+    for (const HalfSpace<T>& h : M.halfspaces) {
+      T dist = dot(center - h.point, h.normal);
+      if (dist <= S.r) contacts.push_back({h.normal, center - dist * h.normal, i});  // contact_halfspace
+    }
+  }
+  const int nc = (int)contacts.size();
+  if (cone_iters) *cone_iters = 0;
+  if (nc == 0) {
+    for (int k = 0; k < nv; ++k) v_next[k] = vfree[k];
+    return 0;
+  }
+  // contact Jacobians C X H (compose_contact_jacobian, dynamics.rs:747-820)
+  const int m = 3 * nc;
+  std::vector<T> J(size_t(m) * nv, T(0));
+  for (int ci = 0; ci < nc; ++ci) {
+    const V3<T> nn = contacts[ci].n, p = contacts[ci].p;
+    V3<T> t = cross(nn, V3<T>(T(1), T(0), T(0)));
+    if (sqrt(dot(t, t)) == T(0)) t = cross(nn, V3<T>(T(0), T(1), T(0)));
+    t = t / sqrt(dot(t, t));
+    V3<T> b = cross(nn, t);
+    b = b / sqrt(dot(b, b));
+    for (int j = contacts[ci].body; j >= 0; j = M.parent[j])
+      for (int k = 0; k < dof[j]; ++k) {
+        V3<T> pv = Sw[j][k].l - cross(p, Sw[j][k].a);  // [-skew(p) I] (w; v)
+        const int col = M.voff[j] + k;
+        J[size_t(3 * ci) * nv + col] = dot(nn, pv);
+        J[size_t(3 * ci + 1) * nv + col] = dot(t, pv);
+        J[size_t(3 * ci + 2) * nv + col] = dot(b, pv);
+      }
+  }
+  // G = J M^-1 J^T (mass_inv from the LU, dynamics.rs:663-667), g = J v_free
+  std::vector<T> Minv(size_t(nv) * nv), e(nv), col(nv);
+  for (int k = 0; k < nv; ++k) {
+    for (int r = 0; r < nv; ++r) e[r] = T(r == k ? 1 : 0);
+    lu_solve(lu, e.data(), col.data());
+    for (int r = 0; r < nv; ++r) Minv[size_t(r) * nv + k] = col[r];
+  }
+  std::vector<T> JM(size_t(m) * nv, T(0)), G(size_t(m) * m, T(0)), g(m, T(0));
+  for (int r = 0; r < m; ++r)
+    for (int k = 0; k < nv; ++k) {
+      T s = T(0);
+      for (int l = 0; l < nv; ++l) s = s + J[size_t(r) * nv + l] * Minv[size_t(l) * nv + k];
+      JM[size_t(r) * nv + k] = s;
+    }
+  for (int r = 0; r < m; ++r) {
+    for (int k = 0; k < m; ++k) {
+      T s = T(0);
+      for (int l = 0; l < nv; ++l) s = s + JM[size_t(r) * nv + l] * J[size_t(k) * nv + l];
+      G[size_t(r) * m + k] = s;
+    }
+    T s = T(0);
+    for (int l = 0; l < nv; ++l) s = s + J[size_t(r) * nv + l] * vfree[l];
+    g[r] = s;
+  }
+  std::vector<T> lam;
+  if (cone_program(G, g, nc, lam, cone_iters)) return 3;
+  std::vector<T> imp(nv, T(0)), dv(nv);
+  for (int k = 0; k < nv; ++k)
+    for (int r = 0; r < m; ++r) imp[k] = imp[k] + J[size_t(r) * nv + k] * lam[r];
+  lu_solve(lu, imp.data(), dv.data());
+  for (int k = 0; k < nv; ++k) v_next[k] = vfree[k] + dv[k];
+  return 0;
+}
+
+// velocity_stepping / ccd_velocity_stepping (integrators.rs:42-175). The
+// continuous-collision loop cuts the step at the earliest time a separated
+// contact point, or a sphere whose body has no contact, reaches a half-space
+// with the new velocities; it gives up (returns 4) after kCcdMaxRounds rounds,
+// where the reference would keep looping.
+constexpr int kCcdMaxRounds = 64;
+
+template <class T>
+int vs_step(const Model<T>& M, bool ccd, const T* q, const T* v, const T* tau, T dt, T* q_out, T* v_out,
+            int* rounds_out = nullptr) {
+  const int nq = M.nq, nv = M.nv, n = M.n;
+  std::vector<T> qc(q, q + nq), vc(v, v + nv), vn(nv), qn(nq);
+  std::vector<VsContact<T>> contacts;
+  T t_remain = dt;
+  for (int round = 0; round < kCcdMaxRounds; ++round) {
+    if (rounds_out) *rounds_out = round + 1;
+    int st = vs_dynamics(M, qc.data(), vc.data(), tau, t_remain, vn.data(), contacts);
+    if (st) return st;
+    T earliest = T(1e300);
+    if (ccd) {
+      std::vector<Iso<T>> b2r = bodies_to_root(M, qc.data());
+      std::vector<SV<T>> tw(n);
+      for (int i = 0; i < n; ++i) {  // compute_twists_wrt_world with the new joint twists
+        const T* vi = vn.data() + M.voff[i];
+        SV<T> j;
+        switch (M.jtype[i]) {
+          case GP_JOINT_REVOLUTE: j.a = M.axis[i] * vi[0]; break;
+          case GP_JOINT_PRISMATIC: j.l = M.axis[i] * vi[0]; break;
+          case GP_JOINT_FLOATING: j.a = V3<T>(vi[0], vi[1], vi[2]); j.l = V3<T>(vi[3], vi[4], vi[5]); break;
+          default: break;
+        }
+        SV<T> parent_tw = M.parent[i] < 0 ? SV<T>() : tw[M.parent[i]];
+        tw[i] = parent_tw + twist_transform(j, b2r[i]);
+      }
+      auto hit = [&](const V3<T>& p, const V3<T>& vel, T radius) {
+        for (const HalfSpace<T>& h : M.halfspaces) {
+          T vdn = dot(vel, h.normal);
+          if (vdn == T(0)) continue;
+          T dist = dot(p - h.point, h.normal) - radius;
+          if (dist <= T(0)) continue;
+          T t = dist / -vdn;
+          if (t < earliest && t > T(0)) earliest = t;
+        }
+      };
+      for (int i = 0; i < n; ++i) {
+        M3<T> R = qmat(b2r[i].r);
+        for (const Contact<T>& cp : M.contacts[i]) {
+          V3<T> pw = R * cp.p + b2r[i].t;
+          hit(pw, tw[i].l + cross(tw[i].a, pw), T(0));
+        }
+      }
+      for (int i = 0; i < n; ++i) {
+        const SphereC<T>& S = M.spheres[i];
+        if (!S.present || !S.enabled) continue;
+        V3<T> center = b2r[i].t + qrot(b2r[i].r, S.c);
+        bool touching = false;
+        for (const VsContact<T>& c : contacts) touching = touching || c.body == i;
+        if (touching) continue;
+        hit(center, tw[i].l + cross(tw[i].a, center), S.r);
+      }
+    }
+    if (earliest < t_remain) {
+      advance_q(M, qc.data(), vn.data(), earliest, qn.data());
+      qc = qn;
+      vc = vn;
+      t_remain = t_remain - earliest;
+    } else {
+      advance_q(M, qc.data(), vn.data(), t_remain, q_out);
+      for (int k = 0; k < nv; ++k) v_out[k] = vn[k];
+      return 0;
+    }
+  }
+  return 4;
 }
 
 // ---------------------------------------------------------------- energy (metric)

@@ -31,6 +31,8 @@ class Oracle:
         lib.oracle_mass_matrix.argtypes = [P, _D, _D, _D, _D]
         lib.oracle_step.argtypes = [P, C.c_int, C.c_int, _D, _D, _D, C.c_double, _D, _D, _D]
         lib.oracle_step_spring.argtypes = [P, _D, _D, _D, C.c_double, _D, _D, _D, _D]
+        lib.oracle_vs_step.argtypes = [P, C.c_int, _D, _D, _D, C.c_double, _D, _D, C.POINTER(C.c_int)]
+        lib.oracle_cone_program.argtypes = [C.c_int, _D, _D, _D, C.POINTER(C.c_int)]
         lib.oracle_rollout.argtypes = [C.POINTER(P), C.c_int64, _D, _D, _D, C.c_double, C.c_int, C.c_int,
                                        C.c_int64, C.c_int, _D]
         lib.oracle_simulate.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_int, _D, _D, _D, _D, _D,
@@ -99,6 +101,31 @@ class Oracle:
         if st:
             raise RuntimeError("oracle: singular mass matrix" if st == 1 else "oracle: spring pulls into the half-space")
         return qo, vo, ao
+
+    def vs_step(self, d, q, v, tau=None, dt=1e-3, ccd=False):
+        """One velocity-stepping step (ccd: with continuous collision
+        detection). Returns (q', v', rounds of the collision loop)."""
+        nq, nv = self.dims(d)
+        q = np.ascontiguousarray(q, float); v = np.ascontiguousarray(v, float)
+        t = np.zeros(nv) if tau is None or len(tau) == 0 else np.ascontiguousarray(tau, float)
+        qo, vo, rounds = np.empty(nq), np.empty(nv), C.c_int()
+        st = self.lib.oracle_vs_step(self._desc(d), int(bool(ccd)), _dp(q), _dp(v), _dp(t), dt, _dp(qo), _dp(vo),
+                                     C.byref(rounds))
+        if st:
+            raise RuntimeError({1: "oracle: singular mass matrix", 3: "oracle: no finite contact impulse",
+                                4: "oracle: collision loop did not finish"}.get(st, f"oracle: status {st}"))
+        return qo, vo, rounds.value
+
+    def cone_program(self, G, g):
+        """min 1/2 x^T G x + g^T x, (0.95 x_n, x_t1, x_t2) in the second-order
+        cone per contact. Returns (x, iterations)."""
+        G = np.ascontiguousarray(G, float); g = np.ascontiguousarray(g, float)
+        nc = g.size // 3
+        x, it = np.empty(g.size), C.c_int()
+        st = self.lib.oracle_cone_program(nc, _dp(G), _dp(g), _dp(x), C.byref(it))
+        if st:
+            raise RuntimeError("oracle: cone program failed")
+        return x, it.value
 
     def rollout(self, descs, q, v, tau, dt, steps, integrator=0, method=CRBA, threads=1, want_vdot=False):
         """descs: one ModelDesc per instance; flattened state; returns new

@@ -132,8 +132,8 @@ def test_struct_layouts_match_the_header(gp, tmp_path):
     """The ctypes mirrors have the sizes and field offsets that a C compiler
     gives the header's structs."""
     import subprocess
-    structs = ["gp_body_desc", "gp_contact_desc", "gp_halfspace_desc", "gp_spring_contact_desc", "gp_model_desc",
-               "gp_batch_opts", "gp_server_opts"]
+    structs = ["gp_body_desc", "gp_contact_desc", "gp_halfspace_desc", "gp_spring_contact_desc", "gp_sphere_desc",
+               "gp_model_desc", "gp_batch_opts", "gp_server_opts"]
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gorilla_amd.h"', 'int main(void) {']
     for s in structs:
         py = getattr(gp.abi, s)
