@@ -14,6 +14,8 @@
 // D = its diagonal entry.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace gp {
@@ -39,6 +41,9 @@ enum HalfspaceField { HF_POINT = 0, HF_NORMAL = 3, HF_ALPHA = 6, HF_MU = 7, NHF 
 // flag, registered world point, half-space index); both [field][nsp][Bp]
 enum SpringField { SF_DIR = 0, SF_L = 3, SF_K = 4, SF_AT = 5, NSF = 14 };
 enum SpringState { SS_REG = 0, SS_P = 1, SS_H = 4, NSS = 5 };
+// sphere colliders (at most one per body): centre in the aligned body frame,
+// radius, enabled (1 / 0); [field][ns][Bp]
+enum SphereField { SH_C = 0, SH_R = 3, SH_ON = 4, NSH = 5 };
 
 // per-lane scratch fields of the runtime-topology (plain) kernel, per body
 enum PlainField {
@@ -56,6 +61,31 @@ enum PlainField {
   PF_TW = 79,   // 3 world translation
   NPF = 82
 };
+
+// Per-lane scratch of the velocity-stepping kernel, after the plain kernel's
+// per-body fields; offsets in units of Bp doubles ([k][Bp] columns). cmax =
+// every (contact point or sphere, half-space) pair; m = 3 cmax impulse rows.
+struct VsLayout {
+  int cmax, m;
+  size_t vfree, zero, col, out, qc, vc, vn, gv, y, j, g, l, x, z, s, rd, lt, xb, dx, dz, ds, xi, rhs, w, wi, cn, cp, cb,
+      total;
+};
+__host__ __device__ inline VsLayout vs_layout(int nb, int nq, int nv, int nc, int nh, int ns) {
+  VsLayout L;
+  L.cmax = (nc + ns) * nh;
+  L.m = 3 * L.cmax;
+  const size_t m = (size_t)L.m, c = (size_t)L.cmax;
+  size_t at = (size_t)NPF * nb;
+  auto take = [&](size_t n) { size_t o = at; at += n; return o; };
+  L.vfree = take(nv); L.zero = take(nv); L.col = take(nv); L.out = take(nv);
+  L.qc = take(nq); L.vc = take(nv); L.vn = take(nv); L.gv = take(m);
+  L.y = take((size_t)nv * m); L.j = take(m * nv); L.g = take(m * m); L.l = take(m * m);
+  L.x = take(m); L.z = take(m); L.s = take(m); L.rd = take(m); L.lt = take(m); L.xb = take(m);
+  L.dx = take(m); L.dz = take(m); L.ds = take(m); L.xi = take(m); L.rhs = take(m);
+  L.w = take(9 * c); L.wi = take(9 * c); L.cn = take(3 * c); L.cp = take(3 * c); L.cb = take(c);
+  L.total = at;
+  return L;
+}
 
 constexpr int kMaxBodies = 64;
 constexpr double kInsideMargin = 1e-8;  // halfspace.rs:42
@@ -88,6 +118,10 @@ struct BucketArgs {
   double* sp;          // [NSF][nsp][Bp] (direction and rest length change when a leg detaches)
   double* ss;          // [NSS][nsp][Bp]
   const float* hpf;
+  int ns;              // sphere colliders per instance
+  const int* sphere_of;  // [nb] the body's sphere slot or -1
+  const double* shp;   // [NSH][ns][Bp]
+  int no_springs;      // plain dynamics without the prismatic springs (M^-1 products)
   // state
   double* q;           // [nq][Bp]
   double* v;           // [nv][Bp]
@@ -134,6 +168,10 @@ int launch_fast_step(int topo_id, int precision, const BucketArgs& a, double dt,
 int launch_fast_rk_step(int topo_id, int precision, int integrator, const BucketArgs& a, double dt, int n_steps,
                         int block, void* stream);
 size_t plain_scratch_doubles(int nb, int nq, int nv, int Bp);
+// velocity stepping (ccd = 1: with continuous collision detection), FP64, one
+// step per launch; scratch as vs_layout
+int launch_vs_step(const BucketArgs& a, double dt, int ccd, void* stream);
+size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp);
 // rows x B values between a bucket's SoA rows and a flat instance-major array
 // (lane l's values start at flat + flat_off[l]); to_soa = 1: flat -> SoA.
 int launch_flat_xfer(double* soa, int Bp, int B, int rows, const int64_t* flat_off, double* flat, int to_soa,

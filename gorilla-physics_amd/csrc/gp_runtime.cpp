@@ -255,6 +255,31 @@ int model_create(const gp_model_desc* d, gp_model** out) {
     for (int k = 0; k < 9; ++k) S[SF_AT + k] = At.a[k];
   }
   for (int i = 0; i < n; ++i) m->sstart[i + 1] += m->sstart[i];
+  // sphere colliders (collision/sphere.rs:6-39), at most one per body
+  // (RigidBody::collider is an Option), centres in the aligned body frames
+  if (d->n_spheres < 0 || (d->n_spheres > 0 && !d->spheres)) return fail(GP_ERR_INVALID, "bad sphere colliders");
+  m->sphere_of.assign(n, -1);
+  for (int c = 0; c < d->n_spheres; ++c) {
+    const gp_sphere_desc& sd = d->spheres[c];
+    if (sd.body < 0 || sd.body >= n) return fail(GP_ERR_INVALID, "sphere %d: body %d out of range", c, sd.body);
+    if (m->sphere_of[sd.body] >= 0) return fail(GP_ERR_INVALID, "sphere %d: body %d already has a collider", c, sd.body);
+    if (!finite_all(sd.center, 3) || !std::isfinite(sd.radius) || sd.radius < 0)
+      return fail(GP_ERR_INVALID, "sphere %d: bad parameters", c);
+    m->sphere_of[sd.body] = 0;
+  }
+  m->ns = 0;
+  for (int i = 0; i < n; ++i)
+    if (m->sphere_of[i] >= 0) m->sphere_of[i] = m->ns++;
+  m->shp.assign((size_t)m->ns * NSH, 0.0);
+  for (int c = 0; c < d->n_spheres; ++c) {
+    const gp_sphere_desc& sd = d->spheres[c];
+    double* S = &m->shp[(size_t)m->sphere_of[sd.body] * NSH];
+    double ca[3];
+    mv(mT(Al[sd.body]), sd.center, ca);
+    for (int k = 0; k < 3; ++k) S[SH_C + k] = ca[k];
+    S[SH_R] = sd.radius;
+    S[SH_ON] = sd.enabled ? 1.0 : 0.0;
+  }
   // topology signature: bodies, parents, DOF class, contacts per body, half-spaces
   uint64_t h = 1469598103934665603ull;
   h = fnv(h, n);
@@ -263,6 +288,7 @@ int model_create(const gp_model_desc* d, gp_model** out) {
     h = fnv(h, m->dofcls[i]);
     h = fnv(h, m->cstart[i + 1] - m->cstart[i]);
     h = fnv(h, m->sstart[i + 1] - m->sstart[i]);
+    h = fnv(h, m->sphere_of[i] >= 0 ? 1 : 0);
   }
   h = fnv(h, m->nh);
   m->sig = h;
@@ -283,6 +309,9 @@ void Bucket::release() {
   d_flat_off = nullptr;
   d_sstart = nullptr;
   d_sp = d_ss = nullptr;
+  d_sphere_of = nullptr;
+  d_shp = nullptr;
+  scratch_doubles = 0;
   d_parent = d_qoff = d_voff = d_cstart = d_kind = d_fault = d_active = nullptr;
   d_bp = d_cp = d_hp = d_q = d_v = d_tau = d_tau_tmp = d_vdot = d_scratch = nullptr;
   d_bpf = d_cpf = d_hpf = nullptr;
@@ -295,6 +324,7 @@ void Bucket::release() {
 int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, bool fill_lanes) {
   const gp_model* m0 = lane_models[0];
   b.nb = m0->n; b.nq = m0->nq; b.nv = m0->nv; b.nc = m0->nc; b.nh = m0->nh; b.nsp = m0->nsp; b.g = m0->g;
+  b.ns = m0->ns; b.sphere_of = m0->sphere_of;
   b.parent = m0->parent; b.qoff = m0->qoff; b.voff = m0->voff; b.dofcls = m0->dofcls; b.cstart = m0->cstart;
   b.sstart = m0->sstart; b.spring_slot = m0->spring_slot;
   b.fast_topo = m0->fast_topo;
@@ -316,7 +346,8 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
                o_hp = take(8 * NHF * b.nh * Bp), o_hpf = take(4 * NHF * b.nh * Bp), o_q = take(8 * nq1 * Bp),
                o_v = take(8 * nv1 * Bp), o_tau = take(8 * nv1 * Bp), o_tau_tmp = take(8 * nv1 * Bp),
                o_vdot = take(8 * nv1 * Bp), o_fault = take(4 * Bp), o_sstart = take(4 * (b.nb + 1)),
-               o_sp = take(8 * NSF * b.nsp * Bp), o_ss = take(8 * NSS * b.nsp * Bp);
+               o_sp = take(8 * NSF * b.nsp * Bp), o_ss = take(8 * NSS * b.nsp * Bp),
+               o_sphere_of = take(4 * b.nb), o_shp = take(8 * NSH * b.ns * Bp);
   std::vector<char> h(at, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(h.data() + off, src, bytes); };
   put(o_parent, b.parent.data(), 4 * b.nb);
@@ -324,6 +355,8 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
   put(o_voff, b.voff.data(), 4 * b.nb);
   put(o_cstart, b.cstart.data(), 4 * (b.nb + 1));
   put(o_sstart, b.sstart.data(), 4 * (b.nb + 1));
+  put(o_sphere_of, b.sphere_of.data(), 4 * b.nb);
+  double* shp = (double*)(h.data() + o_shp);
   double* sp = (double*)(h.data() + o_sp);
   int* kind = (int*)(h.data() + o_kind);
   double* bp = (double*)(h.data() + o_bp);
@@ -359,6 +392,8 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
       }
     for (int s = 0; s < b.nsp; ++s)
       for (int f = 0; f < NSF; ++f) sp[((size_t)f * b.nsp + s) * Bp + l] = m->sp[(size_t)s * NSF + f];
+    for (int s = 0; s < b.ns; ++s)
+      for (int f = 0; f < NSH; ++f) shp[((size_t)f * b.ns + s) * Bp + l] = m->shp[(size_t)s * NSH + f];
   }
   // zero state (mechanism.rs:68-88): q = 0, floating q = identity pose, v = 0, tau = 0
   double* q0 = (double*)(h.data() + o_q);
@@ -377,15 +412,26 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
   b.d_q = (double*)(d + o_q); b.d_v = (double*)(d + o_v); b.d_tau = (double*)(d + o_tau);
   b.d_tau_tmp = (double*)(d + o_tau_tmp); b.d_vdot = (double*)(d + o_vdot); b.d_fault = (int*)(d + o_fault);
   b.d_sstart = (int*)(d + o_sstart); b.d_sp = (double*)(d + o_sp); b.d_ss = (double*)(d + o_ss);
+  b.d_sphere_of = (int*)(d + o_sphere_of); b.d_shp = (double*)(d + o_shp);
   b.arena_bytes = at;
   return GP_OK;
 }
 
+// Scratch of the runtime-topology kernels: room for the plain kernel (per-body
+// fields and Runge-Kutta stages) and for the velocity-stepping kernel.
 int bucket_ensure_scratch(Bucket& b) {
-  if (b.d_scratch) return GP_OK;
-  const size_t bytes = sizeof(double) * std::max<size_t>(1, plain_scratch_doubles(b.nb, b.nq, b.nv, b.Bp));
+  const size_t need = std::max<size_t>({(size_t)1, plain_scratch_doubles(b.nb, b.nq, b.nv, b.Bp),
+                                        vs_scratch_doubles(b.nb, b.nq, b.nv, b.nc, b.nh, b.ns, b.Bp)});
+  if (b.d_scratch && b.scratch_doubles >= need) return GP_OK;
+  if (b.d_scratch) {
+    (void)hipFree(b.d_scratch);
+    b.d_scratch = nullptr;
+    b.scratch_doubles = 0;
+  }
+  const size_t bytes = sizeof(double) * need;
   hipError_t e = hipMalloc((void**)&b.d_scratch, bytes);
   if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+  b.scratch_doubles = need;
   return GP_OK;
 }
 
@@ -404,6 +450,7 @@ BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
   a.q = b.d_q; a.v = b.d_v; a.tau = tau ? tau : b.d_tau; a.vdot = b.d_vdot; a.fault = b.d_fault;
   a.active = b.d_active; a.scratch = b.d_scratch;
   a.nsp = b.nsp; a.sstart = b.d_sstart; a.sp = b.d_sp; a.ss = b.d_ss;
+  a.ns = b.ns; a.sphere_of = b.d_sphere_of; a.shp = b.d_shp; a.no_springs = 0;
   return a;
 }
 
@@ -412,8 +459,11 @@ BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
 using namespace gp;
 
 // ------------------------------------------------------------------ batch
+static bool is_velocity_stepping(int integrator) {
+  return integrator == GP_VELOCITY_STEPPING || integrator == GP_CCD_VELOCITY_STEPPING;
+}
 static bool bucket_is_fast(const gp_batch_opts& o, const Bucket& b, int integrator) {
-  return o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0;
+  return o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0 && !is_velocity_stepping(integrator);
 }
 static int bucket_lanes(const gp_batch_opts& o, const Bucket& b, int integrator) {
   // the tree-parallel quadruped kernel is semi-implicit Euler only
@@ -478,6 +528,19 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
     if (o.kernel == GP_KERNEL_FAST && !fast)
       return fail(GP_ERR_UNSUPPORTED, "no FAST kernel for this bucket's topology / integrator");
     if (fast && grouped) continue;
+    if (is_velocity_stepping(integrator)) {
+      if (o.precision == GP_PREC_MIXED && o.kernel == GP_KERNEL_PLAIN)
+        return fail(GP_ERR_UNSUPPORTED, "velocity stepping computes in FP64 only");
+      int st = bucket_ensure_scratch(b);
+      if (st) return st;
+      BucketArgs a = bucket_args(b, 1, nullptr);
+      for (int k = 0; k < n_steps; ++k) {
+        int e = launch_vs_step(a, dt, integrator == GP_CCD_VELOCITY_STEPPING ? 1 : 0, s);
+        if (e) return fail(GP_ERR_DEVICE, "velocity-stepping launch: %s", hipGetErrorString((hipError_t)e));
+        ++*launches;
+      }
+      continue;
+    }
     if (fast) {
       int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
       int lanes = bucket_lanes(o, b, integrator);
@@ -824,10 +887,10 @@ int gp_batch_set_tau(gp_batch* bt, const double* tau) {
 int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* stream) {
   if (!bt) return fail(GP_ERR_INVALID, "null batch");
   if (!std::isfinite(dt)) return fail(GP_ERR_INVALID, "dt is not finite");
-  if (integrator < GP_SEMI_IMPLICIT_EULER || integrator > GP_RUNGE_KUTTA_4)
+  if (integrator < GP_SEMI_IMPLICIT_EULER || integrator > GP_CCD_VELOCITY_STEPPING)
     return fail(GP_ERR_INVALID, "unknown integrator %d", integrator);
   if (n_steps < 0) return fail(GP_ERR_INVALID, "n_steps < 0");
-  if (integrator != GP_SEMI_IMPLICIT_EULER)
+  if (integrator == GP_RUNGE_KUTTA_2 || integrator == GP_RUNGE_KUTTA_4)
     for (const Bucket& b : bt->buckets)
       if (b.nsp > 0)  // simulate.rs:56-71 asserts the same
         return fail(GP_ERR_UNSUPPORTED, "Runge-Kutta on a state with spring contacts");

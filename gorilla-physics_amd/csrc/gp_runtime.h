@@ -26,6 +26,9 @@ struct gp_model {
   std::vector<int> sstart;         // [n + 1] spring contacts sorted by body
   std::vector<double> sp;          // [nsp][NSF]
   std::vector<int> spring_slot;    // index in the description -> sorted slot
+  int ns = 0;
+  std::vector<int> sphere_of;      // [n] sphere collider slot of the body or -1
+  std::vector<double> shp;         // [ns][NSH]
   uint64_t sig = 0;
   int fast_topo = -1;
   bool all_rev = true;  // no prismatic joint
@@ -34,21 +37,25 @@ struct gp_model {
 namespace gp {
 
 struct Bucket {
-  int nb = 0, nq = 0, nv = 0, nc = 0, nh = 0, nsp = 0, B = 0, Bp = 0, fast_topo = -1;
+  int nb = 0, nq = 0, nv = 0, nc = 0, nh = 0, nsp = 0, ns = 0, B = 0, Bp = 0, fast_topo = -1;
   bool all_rev = false;  // every 1-DOF joint of every lane is revolute
   double g = 9.81;
   std::vector<int> parent, qoff, voff, dofcls, cstart, sstart;
   std::vector<int> spring_slot;
+  std::vector<int> sphere_of;
   std::vector<int64_t> inst;  // lane -> instance index
   int *d_parent = nullptr, *d_qoff = nullptr, *d_voff = nullptr, *d_cstart = nullptr, *d_kind = nullptr;
   int* d_sstart = nullptr;
   double *d_sp = nullptr, *d_ss = nullptr;
+  int* d_sphere_of = nullptr;
+  double* d_shp = nullptr;
   double *d_bp = nullptr, *d_cp = nullptr, *d_hp = nullptr;
   float *d_bpf = nullptr, *d_cpf = nullptr, *d_hpf = nullptr;
   double *d_q = nullptr, *d_v = nullptr, *d_tau = nullptr, *d_tau_tmp = nullptr, *d_vdot = nullptr;
   int* d_fault = nullptr;
   int* d_active = nullptr;
   double* d_scratch = nullptr;
+  size_t scratch_doubles = 0;
   void* d_arena = nullptr;  // one allocation holding every array above except scratch and active
   size_t arena_bytes = 0;
   int64_t* d_flat_off = nullptr;  // [2][B]: each lane's q and v offsets in the flattened arrays
@@ -59,11 +66,13 @@ struct Bucket {
   Bucket& operator=(Bucket&& o) noexcept {
     if (this != &o) {
       release();
-      nb = o.nb; nq = o.nq; nv = o.nv; nc = o.nc; nh = o.nh; nsp = o.nsp; B = o.B; Bp = o.Bp; fast_topo = o.fast_topo;
+      nb = o.nb; nq = o.nq; nv = o.nv; nc = o.nc; nh = o.nh; nsp = o.nsp; ns = o.ns; B = o.B; Bp = o.Bp;
+      fast_topo = o.fast_topo;
       g = o.g;
-      sstart = std::move(o.sstart); spring_slot = std::move(o.spring_slot);
-      d_sstart = o.d_sstart; d_sp = o.d_sp; d_ss = o.d_ss;
-      o.d_sstart = nullptr; o.d_sp = o.d_ss = nullptr;
+      sstart = std::move(o.sstart); spring_slot = std::move(o.spring_slot); sphere_of = std::move(o.sphere_of);
+      d_sstart = o.d_sstart; d_sp = o.d_sp; d_ss = o.d_ss; d_sphere_of = o.d_sphere_of; d_shp = o.d_shp;
+      o.d_sstart = nullptr; o.d_sp = o.d_ss = nullptr; o.d_sphere_of = nullptr; o.d_shp = nullptr;
+      scratch_doubles = o.scratch_doubles;
       all_rev = o.all_rev;
       parent = std::move(o.parent); qoff = std::move(o.qoff); voff = std::move(o.voff); dofcls = std::move(o.dofcls);
       cstart = std::move(o.cstart); inst = std::move(o.inst);

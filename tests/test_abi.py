@@ -184,3 +184,27 @@ def test_spring_contact_models(gp):
     bad.desc.spring_contacts = C.POINTER(gp.abi.gp_spring_contact_desc)()
     with pytest.raises(gp.abi.GPError):
         batch.Model(bad)
+
+
+def test_sphere_collider_models(gp):
+    """Sphere colliders (collision/sphere.rs:6-39) are part of the topology
+    signature; a body holds at most one collider; bad radii are refused."""
+    from gorilla_physics_amd import batch, builders
+    st = builders.build_sphere(1.0, 0.5)
+    d = st.model_desc()
+    assert d.desc.n_spheres == 1 and d.desc.spheres[0].radius == 0.5 and d.desc.spheres[0].enabled == 1
+    m = batch.Model(d)
+    bare = builders.build_sphere(1.0, 0.5)
+    bare.bodies[0].collider = None
+    assert batch.Model(bare.model_desc()).signature() != m.signature()
+    for field, value in (("body", 1), ("radius", -1.0), ("radius", float("inf"))):
+        bad = st.model_desc()
+        setattr(bad.desc.spheres[0], field, value)
+        with pytest.raises(gp.abi.GPError) as e:
+            batch.Model(bad)
+        assert e.value.code == gp.abi.GP_ERR_INVALID, field
+    two = (gp.abi.gp_sphere_desc * 2)(d.desc.spheres[0], d.desc.spheres[0])
+    bad = st.model_desc()
+    bad.desc.spheres, bad.desc.n_spheres = C.cast(two, C.POINTER(gp.abi.gp_sphere_desc)), 2
+    with pytest.raises(gp.abi.GPError):
+        batch.Model(bad)

@@ -1,0 +1,252 @@
+"""Velocity stepping (integrators.rs:42-175, dynamics.rs:366-742) through the
+HIP path: every step of every instance checked against the oracle stepped
+from the same state, and the reference's own velocity-stepping and
+continuous-collision tests (collision/sphere.rs, integrators.rs,
+collision/ccd/mod.rs) applied to the GPU trajectories, the random ones run as
+one batch of instances.
+
+Tolerance of the per-step check: 1e-6 relative on q and v. Both sides solve
+the contact-impulse cone program with an interior-point method to a relative
+gap of 1e-16; where a contact slides, the impulse sits on the cone's boundary
+and the method's answer is accurate to about 1e-7 relative (the square root
+of the gap), so rounding differences between the two can move the result by
+that much. The reference's own solver (Clarabel, gap tolerance 1e-8) is
+looser still. Steps without sliding contacts agree to about 1e-12."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+VS, CCD = 3, 4
+TOL = 1e-6
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(1.0, np.abs(b).max())
+
+
+class Stepper:
+    """A batch of instances of several models stepped on the GPU, each step
+    compared with the oracle from the same state."""
+
+    def __init__(self, oracle, descs, q, v, tau=None, integrator=VS):
+        from gorilla_physics_amd import batch
+        self.oracle, self.descs, self.integrator = oracle, descs, integrator
+        models, self._m = [], {}
+        for d in descs:
+            if id(d) not in self._m:
+                self._m[id(d)] = batch.Model(d)
+            models.append(self._m[id(d)])
+        self.b = batch.Batch(models, [1] * len(models), precision=0)
+        self.q, self.v = np.array(q, float), np.array(v, float)
+        self.tau = np.zeros(self.b.nv_total) if tau is None else np.array(tau, float)
+        self.b.set_state(self.q, self.v)
+        self.b.set_tau(self.tau)
+        self.worst = 0.0
+        self.qo = np.cumsum([0] + [d.nq for d in descs])
+        self.vo = np.cumsum([0] + [d.nv for d in descs])
+
+    def step(self, dt, check=True):
+        self.b.step(dt, self.integrator, 1)
+        qn, vn = self.b.get_state()
+        if check:
+            for i, d in enumerate(self.descs):
+                qs, vs = slice(self.qo[i], self.qo[i + 1]), slice(self.vo[i], self.vo[i + 1])
+                rq, rv, _ = self.oracle.vs_step(d, self.q[qs], self.v[vs], self.tau[vs], dt,
+                                                self.integrator == CCD)
+                e = max(_rel(qn[qs], rq), _rel(vn[vs], rv))
+                self.worst = max(self.worst, e)
+                assert e < TOL, (i, e)
+        self.q, self.v = qn, vn
+        return qn, vn
+
+
+def test_teacher_forced_cubes_spheres_and_quadrupeds(gp, oracle):
+    """Random cubes and spheres near a tilted ground (touching, sliding,
+    resting, in flight) and quadrupeds standing on it, in one batch."""
+    from gorilla_physics_amd import builders
+    rng = np.random.default_rng(0)
+    cube = builders.build_cube(1.0, 1.0)
+    sphere = builders.build_sphere(1.0, 0.5)
+    quad = builders.build_quadruped()
+    normal = np.array([math.sin(0.1), 0.0, math.cos(0.1)])
+    for st in (cube, sphere, quad):
+        st.add_halfspace(gp.HalfSpace.new(normal, 0.0))
+    dc, ds, dq = cube.model_desc(), sphere.model_desc(), quad.model_desc()
+    descs, q, v = [], [], []
+    for k in range(48):
+        axis = rng.normal(size=3)
+        rot = gp.UnitQuaternion.from_axis_angle(axis / np.linalg.norm(axis), rng.uniform(0, 2 * math.pi))
+        if k % 3 == 0:
+            descs.append(dc)
+            q += [*rot.coords(), *rng.uniform(-1, 1, 2), rng.uniform(0.3, 1.0)]
+            v += [*rng.normal(size=3), *rng.normal(size=3)]
+        elif k % 3 == 1:
+            descs.append(ds)
+            q += [*rot.coords(), *rng.uniform(-1, 1, 2), rng.uniform(0.45, 0.55)]
+            v += [*rng.normal(size=3), *rng.normal(size=3)]
+        else:
+            descs.append(dq)
+            qq = np.zeros(dq.nq)
+            qq[3], qq[6] = 1.0, rng.uniform(0.9, 1.3)
+            qq[7:] = rng.normal(size=dq.nq - 7) * 0.2
+            q += list(qq)
+            v += list(rng.normal(size=dq.nv) * 0.3)
+    s = Stepper(oracle, descs, q, v)
+    print("case velocity stepping teacher-forced mix of 48", flush=True)
+    for _ in range(40):
+        s.step(1e-2)
+    print(f"worst per-step difference {s.worst:.3g}", flush=True)
+    assert not s.b.faults().any()
+
+
+def test_particle_slide_closed_form(gp, oracle):
+    """A body whose only contact point is its centre of mass, sliding at u on
+    the ground: slip (u - mu g dt) / (1 + mu^2), lift-off speed mu times that."""
+    from gorilla_physics_amd import batch
+    body = gp.RigidBody(gp.SpatialInertia("p", np.eye(3) * 0.1, np.zeros(3), 1.0))
+    st = gp.MechanismState([gp.FloatingJoint(gp.Transform3D.identity("p", gp.WORLD_FRAME))], [body])
+    st.add_contact_point(gp.ContactPoint.new("p", [0, 0, 0]))
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1.0], 0.0))
+    speeds = [0.0, 0.004, 1.0, 3.0]
+    b = batch.Batch([batch.Model(st.model_desc())], [len(speeds)], precision=0)
+    b.set_state(np.tile([0, 0, 0, 1.0, 0, 0, 0], len(speeds)),
+                np.concatenate([[0, 0, 0, u * 0.6, u * 0.8, 0] for u in speeds]))
+    b.step(1e-3, VS, 1)
+    _, v = b.get_state()
+    for k, u in enumerate(speeds):
+        vk = v[6 * k:6 * k + 6]
+        if u > 0.95 * 9.81e-3:
+            s = (u - 0.95 * 9.81e-3) / (1 + 0.95 ** 2)
+            assert abs(np.linalg.norm(vk[3:5]) - s) < 1e-10 and abs(vk[5] - 0.95 * s) < 1e-10
+        else:
+            assert np.abs(vk).max() < 1e-10
+
+
+def test_sphere_hit_ground_and_roll_forward(gp, oracle):  # collision/sphere.rs:58,96
+    from gorilla_physics_amd import builders
+    st = builders.build_sphere(1.0, 1.0)
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1.0], 0.0))
+    r = 0.5
+    st2 = builders.build_sphere(1.0, r)
+    st2.add_halfspace(gp.HalfSpace.new([0, 0, 1.0], 0.0))
+    s = Stepper(oracle, [st.model_desc(), st2.model_desc()], [0, 0, 0, 1.0, 0, 0, 1.5, 0, 0, 0, 1.0, 0, 0, r],
+                np.zeros(12), tau=[0] * 6 + [0, 0.1, 0, 0, 0, 0])
+    print("case reference sphere_hit_ground + sphere_roll_forward", flush=True)
+    for k in range(2000):
+        q, v = s.step(1e-3, check=k % 10 == 0)
+    assert np.abs(q[4:7] - [0, 0, 1]).max() < 1e-3
+    R = gp.UnitQuaternion(*q[7:11]).to_matrix()
+    roll, pitch, yaw = math.atan2(R[2, 1], R[2, 2]), math.asin(-R[2, 0]), math.atan2(R[1, 0], R[0, 0])
+    assert roll == 0.0 and yaw == 0.0 and q[12] == 0.0
+    assert abs(q[13] - r) < 1e-5 and abs(r * pitch - q[11]) < 1e-4
+    print(f"worst per-step difference {s.worst:.3g}", flush=True)
+
+
+def test_shoot_box_and_drop_quadruped(gp, oracle):  # integrators.rs:340,379
+    from gorilla_physics_amd import builders
+    l = 1.0
+    cube = builders.build_cube(1.0, l)
+    rot = gp.UnitQuaternion.from_axis_angle([0, 1.0, 0], math.pi / 2)
+    inv = gp.UnitQuaternion(-rot.x, -rot.y, -rot.z, rot.w)
+    a = math.radians(-10.0)
+    normal = np.array([math.sin(a), 0.0, math.cos(a)])
+    cube.add_halfspace(gp.HalfSpace.new(normal, 0.0))
+    s = Stepper(oracle, [cube.model_desc()], [*rot.coords(), 0, 0, 1.0], [0, 0, 0, *(inv * np.array([1.0, 0, 0]))])
+    print("case reference velocity_stepping_shoot_box", flush=True)
+    for k in range(int(2.0 / (1.0 / 600.0))):
+        q, v = s.step(1.0 / 600.0, check=k % 5 == 0)
+    assert abs(q[4:7] @ normal - l / 2) < 1e-2 and abs(q[5]) < 1e-5 and np.abs(v).max() < 1e-5
+    quad = builders.build_quadruped()
+    quad.add_halfspace(gp.HalfSpace.new([0, 0, 1.0], 0.0))
+    d = quad.model_desc()
+    q0 = np.zeros(d.nq)
+    q0[3], q0[6] = 1.0, 5.0
+    s = Stepper(oracle, [d], q0, np.zeros(d.nv))
+    print("case reference velocity_stepping_drop_quadruped", flush=True)
+    for k in range(300):
+        q, v = s.step(1e-2)
+    assert q[6] > -1e-3 and np.abs(v[:6]).max() < 1e-3
+    print(f"worst per-step difference {s.worst:.3g}", flush=True)
+
+
+@pytest.mark.parametrize("tilt", [0.0, 10.0])
+def test_ccd_spheres(gp, oracle, tilt):  # collision/ccd/mod.rs:38,90,152
+    from gorilla_physics_amd import builders
+    r = 0.1
+    a = math.radians(tilt)
+    normal = np.array([math.sin(a), 0.0, math.cos(a)])
+    one = builders.build_sphere(1.0, r)
+    one.add_halfspace(gp.HalfSpace.new(normal, 0.0))
+    three = builders.build_n_spheres(1.0, r, 3)
+    three.add_halfspace(gp.HalfSpace.new(normal, 0.0))
+    ys = [0.0, 1.0, 2.0]
+    q = [0, 0, 0, 1.0, 0, 0, 10 * r] + sum(([0, 0, 0, 1.0, 0, y, 10 * r + 1e-8 * k] for k, y in enumerate(ys)), [])
+    v = [0, 0, 0, 0, 0, -5.0] * 4
+    s = Stepper(oracle, [one.model_desc(), three.model_desc()], q, v, integrator=CCD)
+    print(f"case reference ccd spheres tilt={tilt}", flush=True)
+    worst_pen = 0.0
+    steps = int((1.0 if tilt == 0 else 0.5) / (1.0 / 60.0))
+    for _ in range(steps):
+        q, v = s.step(1.0 / 60.0)
+        for k in range(4):
+            worst_pen = max(worst_pen, max(0.0, -(q[7 * k + 4:7 * k + 7] @ normal - r)))
+    assert worst_pen < 1e-8
+    if tilt == 0:
+        assert abs(q[6] - r) < 1e-8
+        for k, y in enumerate(ys):
+            assert abs(q[7 * (k + 1) + 6] - r) < 1e-8
+            assert q[7 * (k + 1) + 4] == 0.0 and q[7 * (k + 1) + 5] == y
+    else:
+        R = gp.UnitQuaternion(*q[:4]).to_matrix()
+        vel = R @ v[3:6]
+        assert vel[0] > 1.0 and abs(vel[1]) < 1e-5
+    print(f"worst per-step difference {s.worst:.3g}", flush=True)
+
+
+def test_ccd_cubes_as_one_batch(gp, oracle):  # collision/ccd/mod.rs:229,284
+    """The reference's two random-cube tests (100 resting cubes at multiples
+    of pi/2, 100 thrown cubes) as one batch of 200 instances; every tenth
+    step of every cube checked against the oracle."""
+    from gorilla_physics_amd import batch, builders
+    l = 0.1
+    st = builders.build_cube(1.0, l)
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1.0], 0.0))
+    d = st.model_desc()
+    rng = np.random.default_rng(11)
+    q, v, xy = [], [], []
+    for _ in range(100):
+        x, y = rng.uniform(-100, 100, 2)
+        rot = gp.UnitQuaternion.from_euler_angles(*(rng.choice([-1.0, -0.5, 0.0, 0.5, 1.0]) * math.pi for _ in range(3)))
+        q += [*rot.coords(), x, y, l / 2]
+        v += [0.0] * 6
+        xy.append((x, y))
+    for _ in range(100):
+        x, y = rng.uniform(-100, 100, 2)
+        axis = rng.normal(size=3)
+        rot = gp.UnitQuaternion.from_axis_angle(axis / np.linalg.norm(axis), rng.uniform(0, 2 * math.pi))
+        R = rot.to_matrix()
+        q += [*rot.coords(), x, y, rng.uniform(l, 2 * l)]
+        v += [*(R.T @ rng.uniform(-5, 5, 3)), *(R.T @ rng.uniform(-1, 1, 3))]
+    s = Stepper(oracle, [d] * 200, q, v, integrator=CCD)
+    print("case reference ccd cubes, 200 in one batch", flush=True)
+    dt = 1.0 / 60.0 / 2.0
+    for k in range(int(1.0 / dt)):
+        q, v = s.step(dt, check=k % 10 == 0)
+    z = q.reshape(200, 7)
+    for i in range(100):
+        assert abs(z[i, 6] - l / 2) < 1e-6 and abs(z[i, 4] - xy[i][0]) < 1e-6 and abs(z[i, 5] - xy[i][1]) < 1e-6
+    assert np.abs(z[100:, 6] - l / 2).max() < 1e-6
+    print(f"worst per-step difference {s.worst:.3g}", flush=True)
+    assert not s.b.faults().any()
+
+
+def test_velocity_stepping_rejects_mixed_plain(gp):
+    from gorilla_physics_amd import batch, builders
+    st = builders.build_cube(1.0, 1.0)
+    b = batch.Batch([batch.Model(st.model_desc())], [2], precision=1, kernel=1)
+    with pytest.raises(gp.abi.GPError) as e:
+        b.step(1e-3, VS, 1)
+    assert e.value.code == gp.abi.GP_ERR_UNSUPPORTED
