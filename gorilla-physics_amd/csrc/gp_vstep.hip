@@ -18,7 +18,8 @@
 //  6. v_next = v_free + Y x, q advanced with v_next.
 // With continuous collision detection the step is cut at the earliest time
 // a separated contact point, or the sphere of a body without contacts, would
-// reach a half-space with v_next, and the rest of the step is stepped again.
+// reach a half-space with v_next, and the rest of the step is stepped again;
+// a sphere that ended a cut touches that half-space in the next round.
 //
 // Every per-lane array is a [k][Bp] column block of the bucket's scratch
 // (vs_layout), so each access of a wavefront is one coalesced 512-byte line.
@@ -256,8 +257,12 @@ __device__ bool cone_program(const VsLane& V, int nc) {
 
 // dynamics_discrete for one lane at (q, v) with step h: new velocities in
 // L.vn. Returns false on a non-positive D or a failed impulse program.
-// *n_contacts = the touching contacts (their bodies in L.cb).
-__device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, double h, int* n_contacts) {
+// *n_contacts = the touching contacts (their bodies in L.cb). forced_body /
+// forced_h: a sphere the collision loop has just moved onto half-space
+// forced_h is in contact with it (its distance is zero in exact arithmetic;
+// rounding may leave it some ulps above the plane).
+__device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, double h, int* n_contacts,
+                            int forced_body, int forced_h) {
   const BucketArgs& a = V.a;
   const int lane = V.lane, nv = a.nv, nb = a.nb, Bp = a.Bp;
   const VsLayout& L = V.L;
@@ -307,7 +312,7 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
       const v3<double> pt{hpar(HF_POINT), hpar(HF_POINT + 1), hpar(HF_POINT + 2)};
       const v3<double> n{hpar(HF_NORMAL), hpar(HF_NORMAL + 1), hpar(HF_NORMAL + 2)};
       const double dist = dot(center - pt, n);
-      if (dist <= r) add(n, center - dist * n, i);
+      if (dist <= r || (i == forced_body && hh == forced_h)) add(n, center - dist * n, i);
     }
   }
   *n_contacts = nc;
@@ -384,14 +389,17 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
 
 // earliest time within which a separated contact point, or the sphere of a
 // body without contacts, reaches a half-space at the new velocities
-// (integrators.rs:62-158); world frames of the current q are in scratch
-__device__ double earliest_collision(const VsLane& V, int nc) {
+// (integrators.rs:62-158); world frames of the current q are in scratch.
+// *hit_body / *hit_h: the sphere and half-space of that time (-1: a point).
+__device__ double earliest_collision(const VsLane& V, int nc, int* hit_body, int* hit_h) {
   const BucketArgs& a = V.a;
   const int lane = V.lane, Bp = a.Bp;
   PlainCtx X{a, lane};
   Strided vn = V.col(V.L.vn), cb = V.col(V.L.cb);
   double earliest = 1e300;
-  auto hit = [&](v3<double> p, v3<double> vel, double radius) {
+  *hit_body = -1;
+  *hit_h = -1;
+  auto hit = [&](v3<double> p, v3<double> vel, double radius, int sphere_body) {
     for (int hh = 0; hh < a.nh; ++hh) {
       auto hpar = [&](int f) { return a.hp[((size_t)f * a.nh + hh) * Bp + lane]; };
       const v3<double> pt{hpar(HF_POINT), hpar(HF_POINT + 1), hpar(HF_POINT + 2)};
@@ -401,7 +409,11 @@ __device__ double earliest_collision(const VsLane& V, int nc) {
       const double dist = dot(p - pt, n) - radius;
       if (dist <= 0.0) continue;
       const double t = dist / -vdn;
-      if (t < earliest && t > 0.0) earliest = t;
+      if (t < earliest && t > 0.0) {
+        earliest = t;
+        *hit_body = sphere_body;
+        *hit_h = hh;
+      }
     }
   };
   for (int i = 0; i < a.nb; ++i) {
@@ -427,7 +439,7 @@ __device__ double earliest_collision(const VsLane& V, int nc) {
     for (int cc = a.cstart[i]; cc < a.cstart[i + 1]; ++cc) {
       auto cpar = [&](int f) { return a.cp[((size_t)f * a.nc + cc) * Bp + lane]; };
       const v3<double> pw = mul(Rw, v3<double>{cpar(CF_P), cpar(CF_P + 1), cpar(CF_P + 2)}) + o;
-      hit(pw, v0 + cross(w, pw), 0.0);
+      hit(pw, v0 + cross(w, pw), 0.0, -1);
     }
   }
   for (int i = 0; i < a.nb; ++i) {
@@ -439,7 +451,7 @@ __device__ double earliest_collision(const VsLane& V, int nc) {
     for (int c = 0; c < nc; ++c) touching = touching || (int)cb[c] == i;
     if (touching) continue;
     const v3<double> center = mul(X.ldm(PF_RW, i), v3<double>{spar(SH_C), spar(SH_C + 1), spar(SH_C + 2)}) + X.ld3(PF_TW, i);
-    hit(center, X.ld3(PF_VEL + 3, i) + cross(X.ld3(PF_VEL, i), center), spar(SH_R));
+    hit(center, X.ld3(PF_VEL + 3, i) + cross(X.ld3(PF_VEL, i), center), spar(SH_R), i);
   }
   return earliest;
 }
@@ -459,14 +471,17 @@ __global__ void __launch_bounds__(256) vs_step_kernel(BucketArgs a, double dt, i
   for (int k = 0; k < nv; ++k) vc[k] = v[k];
   bool ok = true, done = false;
   double t_remain = dt;
+  int forced_body = -1, forced_h = -1;
   for (int round = 0; round < kCcdMaxRounds && !done; ++round) {
-    int nc = 0;
-    ok = vs_dynamics(V, qc, vc, tau, t_remain, &nc) && ok;
-    const double earliest = ccd ? earliest_collision(V, nc) : 1e300;
+    int nc = 0, hit_body = -1, hit_h = -1;
+    ok = vs_dynamics(V, qc, vc, tau, t_remain, &nc, forced_body, forced_h) && ok;
+    const double earliest = ccd ? earliest_collision(V, nc, &hit_body, &hit_h) : 1e300;
     if (earliest < t_remain) {
       plain_advance(a, lane, qc, vn, earliest, qc);
       for (int k = 0; k < nv; ++k) vc[k] = vn[k];
       t_remain -= earliest;
+      forced_body = hit_body;
+      forced_h = hit_body >= 0 ? hit_h : -1;
     } else {
       plain_advance(a, lane, qc, vn, t_remain, qc);
       done = true;

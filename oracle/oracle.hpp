@@ -933,9 +933,14 @@ template <class T> std::vector<Iso<T>> bodies_to_root(const Model<T>& M, const T
 // (solve_constraint_and_contact, dynamics.rs:647-742). Returns 0, 1 on a
 // singular mass matrix, 3 when the cone program gives no finite impulse
 // (the reference panics on NaN, dynamics.rs:731-733).
+// forced_body / forced_h: a sphere the continuous-collision loop has just
+// moved onto half-space forced_h; it is in contact (its distance is zero in
+// exact arithmetic, and rounding may leave it some ulps above the plane, where
+// the reference's loop would find the same collision again without moving).
 template <class T>
 int vs_dynamics(const Model<T>& M, const T* q, const T* v, const T* tau, T dt, T* v_next,
-                std::vector<VsContact<T>>& contacts, int* cone_iters = nullptr) {
+                std::vector<VsContact<T>>& contacts, int* cone_iters = nullptr, int forced_body = -1,
+                int forced_h = -1) {
   using std::sqrt;
   const int n = M.n, nv = M.nv;
   std::vector<T> Mm(size_t(nv) * nv), c(nv), scratch(nv), zero(nv, T(0));
@@ -973,9 +978,11 @@ int vs_dynamics(const Model<T>& M, const T* q, const T* v, const T* tau, T dt, T
     const SphereC<T>& S = M.spheres[i];
     if (!S.present || !S.enabled) continue;
     V3<T> center = b2r[i].t + qrot(b2r[i].r, S.c);  // Sphere::center
-    for (const HalfSpace<T>& h : M.halfspaces) {
+    for (size_t hh = 0; hh < M.halfspaces.size(); ++hh) {
+      const HalfSpace<T>& h = M.halfspaces[hh];
       T dist = dot(center - h.point, h.normal);
-      if (dist <= S.r) contacts.push_back({h.normal, center - dist * h.normal, i});  // contact_halfspace
+      if (dist <= S.r || (i == forced_body && int(hh) == forced_h))
+        contacts.push_back({h.normal, center - dist * h.normal, i});  // contact_halfspace
     }
   }
   const int nc = (int)contacts.size();
@@ -1040,8 +1047,9 @@ int vs_dynamics(const Model<T>& M, const T* q, const T* v, const T* tau, T dt, T
 // velocity_stepping / ccd_velocity_stepping (integrators.rs:42-175). The
 // continuous-collision loop cuts the step at the earliest time a separated
 // contact point, or a sphere whose body has no contact, reaches a half-space
-// with the new velocities; it gives up (returns 4) after kCcdMaxRounds rounds,
-// where the reference would keep looping.
+// with the new velocities; a sphere that ended a cut is in contact with that
+// half-space in the next round; the loop gives up (returns 4) after
+// kCcdMaxRounds rounds, where the reference would keep looping.
 constexpr int kCcdMaxRounds = 64;
 
 template <class T>
@@ -1051,11 +1059,13 @@ int vs_step(const Model<T>& M, bool ccd, const T* q, const T* v, const T* tau, T
   std::vector<T> qc(q, q + nq), vc(v, v + nv), vn(nv), qn(nq);
   std::vector<VsContact<T>> contacts;
   T t_remain = dt;
+  int forced_body = -1, forced_h = -1;
   for (int round = 0; round < kCcdMaxRounds; ++round) {
     if (rounds_out) *rounds_out = round + 1;
-    int st = vs_dynamics(M, qc.data(), vc.data(), tau, t_remain, vn.data(), contacts);
+    int st = vs_dynamics(M, qc.data(), vc.data(), tau, t_remain, vn.data(), contacts, nullptr, forced_body, forced_h);
     if (st) return st;
     T earliest = T(1e300);
+    int hit_body = -1, hit_h = -1;
     if (ccd) {
       std::vector<Iso<T>> b2r = bodies_to_root(M, qc.data());
       std::vector<SV<T>> tw(n);
@@ -1071,21 +1081,26 @@ int vs_step(const Model<T>& M, bool ccd, const T* q, const T* v, const T* tau, T
         SV<T> parent_tw = M.parent[i] < 0 ? SV<T>() : tw[M.parent[i]];
         tw[i] = parent_tw + twist_transform(j, b2r[i]);
       }
-      auto hit = [&](const V3<T>& p, const V3<T>& vel, T radius) {
-        for (const HalfSpace<T>& h : M.halfspaces) {
+      auto hit = [&](const V3<T>& p, const V3<T>& vel, T radius, int sphere_body) {
+        for (size_t hh = 0; hh < M.halfspaces.size(); ++hh) {
+          const HalfSpace<T>& h = M.halfspaces[hh];
           T vdn = dot(vel, h.normal);
           if (vdn == T(0)) continue;
           T dist = dot(p - h.point, h.normal) - radius;
           if (dist <= T(0)) continue;
           T t = dist / -vdn;
-          if (t < earliest && t > T(0)) earliest = t;
+          if (t < earliest && t > T(0)) {
+            earliest = t;
+            hit_body = sphere_body;
+            hit_h = int(hh);
+          }
         }
       };
       for (int i = 0; i < n; ++i) {
         M3<T> R = qmat(b2r[i].r);
         for (const Contact<T>& cp : M.contacts[i]) {
           V3<T> pw = R * cp.p + b2r[i].t;
-          hit(pw, tw[i].l + cross(tw[i].a, pw), T(0));
+          hit(pw, tw[i].l + cross(tw[i].a, pw), T(0), -1);
         }
       }
       for (int i = 0; i < n; ++i) {
@@ -1095,7 +1110,7 @@ int vs_step(const Model<T>& M, bool ccd, const T* q, const T* v, const T* tau, T
         bool touching = false;
         for (const VsContact<T>& c : contacts) touching = touching || c.body == i;
         if (touching) continue;
-        hit(center, tw[i].l + cross(tw[i].a, center), S.r);
+        hit(center, tw[i].l + cross(tw[i].a, center), S.r, i);
       }
     }
     if (earliest < t_remain) {
@@ -1103,6 +1118,8 @@ int vs_step(const Model<T>& M, bool ccd, const T* q, const T* v, const T* tau, T
       qc = qn;
       vc = vn;
       t_remain = t_remain - earliest;
+      forced_body = hit_body;
+      forced_h = hit_body >= 0 ? hit_h : -1;
     } else {
       advance_q(M, qc.data(), vn.data(), t_remain, q_out);
       for (int k = 0; k < nv; ++k) v_out[k] = vn[k];

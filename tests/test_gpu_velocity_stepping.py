@@ -45,6 +45,7 @@ class Stepper:
         self.b.set_state(self.q, self.v)
         self.b.set_tau(self.tau)
         self.worst = 0.0
+        self.n = 0
         self.qo = np.cumsum([0] + [d.nq for d in descs])
         self.vo = np.cumsum([0] + [d.nv for d in descs])
 
@@ -58,8 +59,9 @@ class Stepper:
                                                 self.integrator == CCD)
                 e = max(_rel(qn[qs], rq), _rel(vn[vs], rv))
                 self.worst = max(self.worst, e)
-                assert e < TOL, (i, e)
+                assert e < TOL, (self.n, i, e, self.q[qs], self.v[vs], qn[qs], rq, vn[vs], rv)
         self.q, self.v = qn, vn
+        self.n += 1
         return qn, vn
 
 
