@@ -68,14 +68,16 @@ __device__ void world_frames(const BucketArgs& a, int lane) {
 
 __device__ inline double soc_det(double a0, double a1, double a2) { return a0 * a0 - a1 * a1 - a2 * a2; }
 
-// largest step t with u + t du in the cone: first positive root of det
+// largest step t with u + t du in the cone: the first positive root of det,
+// and never past the apex (u0 + t du0 = 0): a direction along the cone's axis
+// has a double root there, which rounding can turn into no real root
 __device__ double soc_max_step(const double* u, const double* du) {
   const double a = soc_det(du[0], du[1], du[2]);
   const double b = u[0] * du[0] - u[1] * du[1] - u[2] * du[2];
   const double c = soc_det(u[0], u[1], u[2]);
-  double best = 1e300;
+  double best = du[0] < 0.0 ? -u[0] / du[0] : 1e300;
   if (a == 0.0) {
-    if (b < 0.0) best = -c / (2.0 * b);
+    if (b < 0.0) best = fmin(best, -c / (2.0 * b));
     return best;
   }
   const double disc = b * b - a * c;
@@ -106,7 +108,7 @@ __device__ void nt_scaling(const double* s, const double* z, double* W, double* 
 
 // The impulse program for nc contacts (G, g in scratch) -> x. Returns false
 // when no finite iterate was produced.
-__device__ bool cone_program(const VsLane& V, int nc) {
+__device__ __noinline__ bool cone_program(const VsLane& V, int nc) {
   const int m = 3 * nc;
   const VsLayout& L = V.L;
   Strided G = V.col(L.g), g = V.col(L.gv), Lc = V.col(L.l), x = V.col(L.x), z = V.col(L.z), s = V.col(L.s),
@@ -261,8 +263,8 @@ __device__ bool cone_program(const VsLane& V, int nc) {
 // forced_h: a sphere the collision loop has just moved onto half-space
 // forced_h is in contact with it (its distance is zero in exact arithmetic;
 // rounding may leave it some ulps above the plane).
-__device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, double h, int* n_contacts,
-                            int forced_body, int forced_h) {
+__device__ __noinline__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, double h,
+                                         int* n_contacts, int forced_body, int forced_h) {
   const BucketArgs& a = V.a;
   const int lane = V.lane, nv = a.nv, nb = a.nb, Bp = a.Bp;
   const VsLayout& L = V.L;
@@ -391,7 +393,7 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
 // body without contacts, reaches a half-space at the new velocities
 // (integrators.rs:62-158); world frames of the current q are in scratch.
 // *hit_body / *hit_h: the sphere and half-space of that time (-1: a point).
-__device__ double earliest_collision(const VsLane& V, int nc, int* hit_body, int* hit_h) {
+__device__ __noinline__ double earliest_collision(const VsLane& V, int nc, int* hit_body, int* hit_h) {
   const BucketArgs& a = V.a;
   const int lane = V.lane, Bp = a.Bp;
   PlainCtx X{a, lane};

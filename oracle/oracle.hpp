@@ -751,15 +751,17 @@ template <class T> void soc_div(const T* u, const T* w, T* y) {
   y[1] = (w[1] - y[0] * u[1]) / u[0];
   y[2] = (w[2] - y[0] * u[2]) / u[0];
 }
-// largest step a with u + a du still in the cone (first positive root of det)
+// largest step a with u + a du still in the cone: the first positive root of
+// det, and never past the apex (u0 + a du0 = 0): a direction along the cone's
+// axis has a double root there, which rounding can turn into no real root
 template <class T> T soc_max_step(const T* u, const T* du) {
   using std::sqrt;
   T a = soc_det(du);
   T b = u[0] * du[0] - u[1] * du[1] - u[2] * du[2];
   T c = soc_det(u);
-  T best = T(1e300);
+  T best = du[0] < T(0) ? -u[0] / du[0] : T(1e300);
   if (a == T(0)) {
-    if (b < T(0)) best = -c / (T(2) * b);
+    if (b < T(0)) best = std::fmin(best, -c / (T(2) * b));
     return best;
   }
   T disc = b * b - a * c;

@@ -59,7 +59,11 @@ class Stepper:
                                                 self.integrator == CCD)
                 e = max(_rel(qn[qs], rq), _rel(vn[vs], rv))
                 self.worst = max(self.worst, e)
-                assert e < TOL, (self.n, i, e, self.q[qs], self.v[vs], qn[qs], rq, vn[vs], rv)
+                if not e < TOL:
+                    print("step", self.n, "instance", i, "error", e, "\nq", self.q[qs].tolist(), "\nv",
+                          self.v[vs].tolist(), "\ngpu q", qn[qs].tolist(), "\noracle q", rq.tolist(), "\ngpu v",
+                          vn[vs].tolist(), "\noracle v", rv.tolist(), "\nfaults", self.b.faults().tolist(), flush=True)
+                assert e < TOL, (self.n, i, e)
         self.q, self.v = qn, vn
         self.n += 1
         return qn, vn
