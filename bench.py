@@ -54,6 +54,8 @@ def make_workload(name, n):
         return workloads.cubes(n=n or 16384)
     if name == "acrobot":
         return workloads.acrobot(n=n or 1024)
+    if name == "thrown_cubes":
+        return workloads.thrown_cubes(n=n or 16384)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -66,19 +68,21 @@ def run_variant(w, models, steps, warmup, autotune=False, **opts):
     b.set_state(w["q"], w["v"])
     b.set_tau(w["tau"])
     tuned_ms = None
+    integrator = w.get("integrator", 0)
     if autotune:
         # trials long enough that launch-to-launch noise does not pick the options
-        _, tuned_ms = b.autotune(w["dt"], 0, trial_steps=min(steps, 1000))
+        _, tuned_ms = b.autotune(w["dt"], integrator, trial_steps=min(steps, 1000))
     o = b.get_opts()
     chosen = dict(block_size=o.block_size, lanes_per_mech=o.lanes_per_mech, lanes_per_tree=o.lanes_per_tree,
                   steps_per_launch=o.steps_per_launch, use_graph=o.use_graph)
     if warmup:
-        b.step(w["dt"], 0, warmup)
+        b.step(w["dt"], integrator, warmup)
     b.sync()
     b.set_state(w["q"], w["v"])
+    b.clear_faults()
     b.sync()
     t0 = time.perf_counter()
-    b.step(w["dt"], 0, steps)
+    b.step(w["dt"], integrator, steps)
     b.sync()
     t1 = time.perf_counter()
     lib_ms, launches = b.last_timing()
@@ -208,18 +212,28 @@ def cpu_baseline(w, budget_s=12.0):
     n = min(n_all, 64 * cores)
     steps = 50
     t0 = time.perf_counter()
-    o.rollout(descs[:n], w["q"][:qo[n]], w["v"][:vo[n]], w["tau"][:vo[n]], w["dt"], steps, threads=cores)
+    integrator = w.get("integrator", 0)
+    o.rollout(descs[:n], w["q"][:qo[n]], w["v"][:vo[n]], w["tau"][:vo[n]], w["dt"], steps, integrator=integrator,
+              threads=cores)
     dt_cal = max(time.perf_counter() - t0, 1e-6)
     rate = n * steps / dt_cal
     target = rate * budget_s
     n = int(min(n_all, max(cores, target // w["steps"])))
     steps = int(min(w["steps"], max(1, target // n)))
     t0 = time.perf_counter()
-    o.rollout(descs[:n], w["q"][:qo[n]], w["v"][:vo[n]], w["tau"][:vo[n]], w["dt"], steps, threads=cores)
+    o.rollout(descs[:n], w["q"][:qo[n]], w["v"][:vo[n]], w["tau"][:vo[n]], w["dt"], steps, integrator=integrator,
+              threads=cores)
     sec = time.perf_counter() - t0
+    algo = ("velocity stepping: CRBA + LU free velocity, interior-point contact impulses"
+            if integrator >= abi_vs() else "CRBA+RNEA+LU")
     return dict(value=n * steps / sec, unit="env-steps/s", cores=cores, kind="port",
-                sample=f"{n} of {n_all} instances x {steps} steps ({sec:.1f} s), CRBA+RNEA+LU FP64 oracle (g++ -O3), "
+                sample=f"{n} of {n_all} instances x {steps} steps ({sec:.1f} s), {algo} FP64 oracle (g++ -O3), "
                        f"{cores} threads of {os.cpu_count()} on {cpu_model()}")
+
+
+def abi_vs():
+    from gorilla_physics_amd import abi
+    return abi.GP_VELOCITY_STEPPING
 
 
 def main():
@@ -279,6 +293,9 @@ def main():
     if args.steps <= 0:
         args.steps = int(w["steps"])
     single = args.workload in ("arms", "acrobot")
+    vstep = w.get("integrator", 0) >= abi.GP_VELOCITY_STEPPING
+    if vstep:  # one step per launch on the runtime-topology kernel: nothing to tune or compare
+        args.autotune, args.no_compare = 0, True
     if args.autotune < 0:
         args.autotune = 0 if single else 1
     if single and not args.autotune:
@@ -289,8 +306,8 @@ def main():
         args.warmup = spl
     common = dict(block_size=args.block, lanes_per_mech=args.lanes, lanes_per_tree=args.lanes_tree,
                   steps_per_launch=spl, use_graph=args.graph, autotune=bool(args.autotune))
-    mixed = run_variant(w, models, args.steps, args.warmup, precision=abi.GP_PREC_MIXED, kernel=abi.GP_KERNEL_AUTO,
-                        **common)
+    mixed = run_variant(w, models, args.steps, args.warmup,
+                        precision=abi.GP_PREC_F64 if vstep else abi.GP_PREC_MIXED, kernel=abi.GP_KERNEL_AUTO, **common)
     log("mixed", mixed)
     extra = {}
     if not args.no_compare:
@@ -332,17 +349,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f64" if vstep else "f32+f64",
         "data": "synthetic (seeded Philox)",
         "config": {"workload": {
             "arms": "65,536 serial 7-DOF revolute arms, dt=1e-3, semi-implicit Euler, FP32 sweeps + FP64 D/q/v",
             "mixed": "32,768 chains 1-12 DOF + floating quadrupeds, dt=1e-3, semi-implicit Euler, mixed precision",
             "cubes": "16,384 free cubes x 8 corner contacts on z=0, dt=1e-4, semi-implicit Euler, mixed precision",
-            "acrobot": "1,024 acrobots, dt=1e-3, semi-implicit Euler, mixed precision"}[args.workload],
+            "acrobot": "1,024 acrobots, dt=1e-3, semi-implicit Euler, mixed precision",
+            "thrown_cubes": "16,384 cubes (side 0.1) thrown at z=0, dt=1/120, velocity stepping with continuous "
+                            "collision detection (cone-program contact impulses), FP64"}[args.workload],
             "instances": mixed["instances"], "steps_per_launch": mixed["opts"]["steps_per_launch"] or spl,
             "hip_graph": bool(args.graph), "launch_opts": mixed["opts"], "autotuned": bool(args.autotune),
-            "error_bound": "teacher-forced per-step vs FP64 oracle: e(q') <= 1e-5, e(v'), e(qdd) <= 2e-3, "
-                           "p99 e(qdd) <= 1e-4 (FP64 kernels: all <= 1e-10), tests/test_gpu_parity.py"},
+            "error_bound": ("teacher-forced per-step vs FP64 oracle: e(q'), e(v') <= 1e-6 (contact impulses from an "
+                            "interior-point method), tests/test_gpu_velocity_stepping.py") if vstep else
+                           ("teacher-forced per-step vs FP64 oracle: e(q') <= 1e-5, e(v'), e(qdd) <= 2e-3, "
+                            "p99 e(qdd) <= 1e-4 (FP64 kernels: all <= 1e-10), tests/test_gpu_parity.py")},
         "roofline": roof,
         "cpu_baseline": cpu,
         "faults": mixed["faults"],

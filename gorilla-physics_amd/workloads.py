@@ -21,7 +21,7 @@ from . import abi
 from .mechanism import (GRAVITY, Isometry3, ModelDesc, SpatialInertia, UnitQuaternion, body_desc, contact_desc,
                         halfspace_desc, HalfSpace)
 
-WORKLOAD_IDS = {"acrobot": 1, "arms": 2, "mixed": 3, "cubes": 4}
+WORKLOAD_IDS = {"acrobot": 1, "arms": 2, "mixed": 3, "cubes": 4, "thrown_cubes": 5}
 
 
 def stream(seed, workload, field):
@@ -158,6 +158,36 @@ def cube_desc(k_point=12500.0, alpha=1.0 / 150.0, mu=0.5, m=1.0, l=1.0):
     contacts = [contact_desc(0, p, k_point) for p in corners]
     hs = [halfspace_desc(HalfSpace.new_with_params([0.0, 0.0, 1.0], 0.0, alpha, mu))]
     return ModelDesc(bodies, contacts, hs)
+
+
+def thrown_cubes(n=16384, seed=0):
+    """Cubes of side 0.1 thrown at a plane and stepped by velocity stepping with
+    continuous collision detection, as the reference's random-cube test
+    (collision/ccd/mod.rs:284): positions U(-100, 100)^2 x U(l, 2l), uniform
+    random rotation, linear velocity U(-1, 1)^3, angular U(-5, 5)^3 (world
+    frame), dt = 1/120, 120 steps (one second)."""
+    from . import builders
+    from .mechanism import HalfSpace
+    l = 0.1
+    st = builders.build_cube(1.0, l)
+    st.add_halfspace(HalfSpace.new([0, 0, 1.0], 0.0))
+    d = st.model_desc()
+    qr = stream(seed, "thrown_cubes", 1).normal(size=(n, 4))
+    qr /= np.linalg.norm(qr, axis=1, keepdims=True)
+    pos = stream(seed, "thrown_cubes", 2).uniform(-100.0, 100.0, (n, 3))
+    pos[:, 2] = stream(seed, "thrown_cubes", 3).uniform(l, 2 * l, n)
+    lin = stream(seed, "thrown_cubes", 4).uniform(-1.0, 1.0, (n, 3))
+    ang = stream(seed, "thrown_cubes", 5).uniform(-5.0, 5.0, (n, 3))
+    q = np.zeros((n, 7))
+    v = np.zeros((n, 6))
+    for i in range(n):
+        R = UnitQuaternion(*qr[i]).to_matrix()
+        q[i, :4] = qr[i]
+        q[i, 4:] = pos[i]
+        v[i, :3] = R.T @ ang[i]
+        v[i, 3:] = R.T @ lin[i]
+    return dict(models=[d], counts=[n], q=q.reshape(-1), v=v.reshape(-1), tau=np.zeros(6 * n),
+                dt=1.0 / 120.0, steps=120, name="thrown_cubes", integrator=4)
 
 
 def cubes(n=16384, seed=0):

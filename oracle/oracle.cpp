@@ -187,7 +187,10 @@ int oracle_rollout(const gp_model_desc* const* descs, int64_t n_inst, double* q,
       };
       for (int64_t s = 0; s < n_steps; ++s) {
         double* ad = (vdot_last && s == n_steps - 1) ? vdot_last + vo[i] : nullptr;
-        if (step_integrate(P.M, integrator, qi, vi, ti, dt, qn.data(), vn.data(), ad, dyn)) {
+        const int st = integrator >= GP_VELOCITY_STEPPING
+                           ? vs_step(P.M, integrator == GP_CCD_VELOCITY_STEPPING, qi, vi, ti, dt, qn.data(), vn.data())
+                           : step_integrate(P.M, integrator, qi, vi, ti, dt, qn.data(), vn.data(), ad, dyn);
+        if (st) {
           failures.fetch_add(1);
           break;
         }

@@ -314,3 +314,18 @@ def test_ccd_cube_ground_dynamic(gp, oracle):  # collision/ccd/mod.rs:284
         q, _ = _run(oracle, d, np.array([*rot.coords(), x, y, z]), v, int(1.0 / (1 / 60.0 / 2.0)), 1 / 60.0 / 2.0,
                     ccd=True)
         assert abs(q[6] - l / 2) < 1e-6
+
+
+def test_rollout_runs_velocity_stepping(gp, oracle):
+    """oracle_rollout with the velocity-stepping integrators is the per-step
+    loop, instance by instance (the CPU baseline of the thrown-cube workload)."""
+    from gorilla_physics_amd import workloads
+    w = workloads.thrown_cubes(n=6)
+    d = w["models"][0]
+    q, v, fails = oracle.rollout([d] * 6, w["q"], w["v"], w["tau"], w["dt"], 30, integrator=4, threads=3)
+    assert fails == 0
+    for i in range(6):
+        qi, vi = w["q"][7 * i:7 * i + 7], w["v"][6 * i:6 * i + 6]
+        for _ in range(30):
+            qi, vi, _ = oracle.vs_step(d, qi, vi, None, w["dt"], True)
+        assert np.array_equal(qi, q[7 * i:7 * i + 7]) and np.array_equal(vi, v[6 * i:6 * i + 6])
