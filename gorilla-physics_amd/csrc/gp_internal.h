@@ -67,8 +67,8 @@ enum PlainField {
 // every (contact point or sphere, half-space) pair; m = 3 cmax impulse rows.
 struct VsLayout {
   int cmax, m;
-  size_t vfree, zero, col, out, qc, vc, vn, gv, y, j, g, l, x, z, s, rd, lt, xb, dx, dz, ds, xi, rhs, w, wi, cn, cp, cb,
-      total;
+  size_t vfree, zero, col, out, qc, vc, vn, gv, mm, y, j, g, l, x, z, s, rd, lt, xb, dx, dz, ds, xi, rhs, w, wi, cn, cp,
+      cb, total;
 };
 __host__ __device__ inline VsLayout vs_layout(int nb, int nq, int nv, int nc, int nh, int ns) {
   VsLayout L;
@@ -79,6 +79,7 @@ __host__ __device__ inline VsLayout vs_layout(int nb, int nq, int nv, int nc, in
   auto take = [&](size_t n) { size_t o = at; at += n; return o; };
   L.vfree = take(nv); L.zero = take(nv); L.col = take(nv); L.out = take(nv);
   L.qc = take(nq); L.vc = take(nv); L.vn = take(nv); L.gv = take(m);
+  L.mm = take((size_t)nv * nv);
   L.y = take((size_t)nv * m); L.j = take(m * nv); L.g = take(m * m); L.l = take(m * m);
   L.x = take(m); L.z = take(m); L.s = take(m); L.rd = take(m); L.lt = take(m); L.xb = take(m);
   L.dx = take(m); L.dz = take(m); L.ds = take(m); L.xi = take(m); L.rhs = take(m);

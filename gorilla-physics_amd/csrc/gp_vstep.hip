@@ -108,7 +108,7 @@ __device__ void nt_scaling(const double* s, const double* z, double* W, double* 
 
 // The impulse program for nc contacts (G, g in scratch) -> x. Returns false
 // when no finite iterate was produced.
-__device__ __noinline__ bool cone_program(const VsLane& V, int nc) {
+__device__ bool cone_program(const VsLane& V, int nc) {
   const int m = 3 * nc;
   const VsLayout& L = V.L;
   Strided G = V.col(L.g), g = V.col(L.gv), Lc = V.col(L.l), x = V.col(L.x), z = V.col(L.z), s = V.col(L.s),
@@ -257,14 +257,125 @@ __device__ __noinline__ bool cone_program(const VsLane& V, int nc) {
   return ok;
 }
 
+// motion subspace column k of body i's joint in world Plücker coordinates
+// (angular w, linear velocity of the point at the world origin)
+__device__ inline void world_column(int kind, const m3<double>& Rw, v3<double> o, int k, v3<double>& w,
+                                    v3<double>& v0) {
+  if (kind == K_REV) {
+    w = mcol(Rw, 2);
+    v0 = cross(o, w);
+  } else if (kind == K_PRI) {
+    w = v3<double>{0, 0, 0};
+    v0 = mcol(Rw, 2);
+  } else if (k < 3) {
+    w = mcol(Rw, k);
+    v0 = cross(o, w);
+  } else {
+    w = v3<double>{0, 0, 0};
+    v0 = mcol(Rw, k - 3);
+  }
+}
+
+// Mass matrix by the composite-rigid-body algorithm in world coordinates
+// (the quantity of mechanism.rs:637-696), then its Cholesky factor, lower
+// triangle in place (row-major nv x nv). World frames of the current q must be
+// in scratch. Composite inertias (J about the world origin, m c, m) are kept
+// in the first ten articulated-inertia fields, free at this point.
+__device__ bool mass_matrix_cholesky(const VsLane& V, Strided Mm) {
+  const BucketArgs& a = V.a;
+  const int lane = V.lane, nb = a.nb, nv = a.nv, Bp = a.Bp;
+  PlainCtx X{a, lane};
+  for (int i = 0; i < nb; ++i) {  // world inertias (inertia.rs:106-134)
+    const m3<double> R = X.ldm(PF_RW, i);
+    const v3<double> p = X.ld3(PF_TW, i);
+    const double J[9] = {X.bpar(BF_J, i), X.bpar(BF_J + 3, i), X.bpar(BF_J + 4, i),
+                         X.bpar(BF_J + 3, i), X.bpar(BF_J + 1, i), X.bpar(BF_J + 5, i),
+                         X.bpar(BF_J + 4, i), X.bpar(BF_J + 5, i), X.bpar(BF_J + 2, i)};
+    const double mass = X.bpar(BF_M, i);
+    const v3<double> Rmc = mul(R, v3<double>{X.bpar(BF_C, i), X.bpar(BF_C + 1, i), X.bpar(BF_C + 2, i)});
+    const v3<double> mp = mass * p;
+    double RJ[9], Jw[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) RJ[3 * r + c] = R.a[3 * r] * J[c] + R.a[3 * r + 1] * J[3 + c] + R.a[3 * r + 2] * J[6 + c];
+    const double rm[3] = {Rmc.x, Rmc.y, Rmc.z}, pp[3] = {p.x, p.y, p.z}, mpp[3] = {mp.x, mp.y, mp.z};
+    double Y[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) Y[3 * r + c] = rm[r] * pp[c] + rm[c] * pp[r] + mpp[r] * pp[c];
+    const double trY = Y[0] + Y[4] + Y[8];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c)
+        Jw[3 * r + c] = RJ[3 * r] * R.a[3 * c] + RJ[3 * r + 1] * R.a[3 * c + 1] + RJ[3 * r + 2] * R.a[3 * c + 2] -
+                        Y[3 * r + c] + (r == c ? trY : 0.0);
+    X.s(PF_IA, i) = Jw[0]; X.s(PF_IA + 1, i) = Jw[4]; X.s(PF_IA + 2, i) = Jw[8];
+    X.s(PF_IA + 3, i) = Jw[1]; X.s(PF_IA + 4, i) = Jw[2]; X.s(PF_IA + 5, i) = Jw[5];
+    X.st3(PF_IA + 6, i, Rmc + mp);
+    X.s(PF_IA + 9, i) = mass;
+  }
+  for (int i = nb - 1; i > 0; --i) {  // composite inertias, leaves to root
+    const int pr = a.parent[i];
+    if (pr < 0) continue;
+    for (int f = 0; f < 10; ++f) X.s(PF_IA + f, pr) += X.s(PF_IA + f, i);
+  }
+  for (int r = 0; r < nv; ++r)
+    for (int c = 0; c <= r; ++c) Mm[r * nv + c] = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    const int ki = a.kind[i * Bp + lane], ni = ki == K_FLOAT ? 6 : (ki == K_FIXED ? 0 : 1);
+    if (ni == 0) continue;
+    const double Jxx = X.s(PF_IA, i), Jyy = X.s(PF_IA + 1, i), Jzz = X.s(PF_IA + 2, i);
+    const double Jxy = X.s(PF_IA + 3, i), Jxz = X.s(PF_IA + 4, i), Jyz = X.s(PF_IA + 5, i);
+    const v3<double> c = X.ld3(PF_IA + 6, i);
+    const double mass = X.s(PF_IA + 9, i);
+    const m3<double> Ri = X.ldm(PF_RW, i);
+    const v3<double> oi = X.ld3(PF_TW, i);
+    for (int ca = 0; ca < ni; ++ca) {
+      v3<double> sw, sv;
+      world_column(ki, Ri, oi, ca, sw, sv);
+      // F = Ic s (momentum.rs:17-32)
+      const v3<double> Jw{Jxx * sw.x + Jxy * sw.y + Jxz * sw.z, Jxy * sw.x + Jyy * sw.y + Jyz * sw.z,
+                          Jxz * sw.x + Jyz * sw.y + Jzz * sw.z};
+      const v3<double> Fa = Jw + cross(c, sv), Fl = mass * sv - cross(c, sw);
+      const int row = a.voff[i] + ca;
+      for (int j = i; j >= 0; j = a.parent[j]) {
+        const int kj = a.kind[j * Bp + lane], nj = kj == K_FLOAT ? 6 : (kj == K_FIXED ? 0 : 1);
+        const m3<double> Rj = X.ldm(PF_RW, j);
+        const v3<double> oj = X.ld3(PF_TW, j);
+        for (int cb2 = 0; cb2 < nj; ++cb2) {
+          const int col = a.voff[j] + cb2;
+          if (col > row) continue;
+          v3<double> tw, tv;
+          world_column(kj, Rj, oj, cb2, tw, tv);
+          Mm[row * nv + col] = dot(Fa, tw) + dot(Fl, tv);
+        }
+      }
+    }
+  }
+  bool pd = true;
+  for (int j = 0; j < nv; ++j) {
+    double t = Mm[j * nv + j];
+    for (int k = 0; k < j; ++k) t -= Mm[j * nv + k] * Mm[j * nv + k];
+    if (!(t > 0.0)) {
+      pd = false;
+      t = 1.0;
+    }
+    const double ljj = sqrt(t);
+    Mm[j * nv + j] = ljj;
+    for (int r = j + 1; r < nv; ++r) {
+      double u = Mm[r * nv + j];
+      for (int k = 0; k < j; ++k) u -= Mm[r * nv + k] * Mm[j * nv + k];
+      Mm[r * nv + j] = u / ljj;
+    }
+  }
+  return pd;
+}
+
 // dynamics_discrete for one lane at (q, v) with step h: new velocities in
 // L.vn. Returns false on a non-positive D or a failed impulse program.
 // *n_contacts = the touching contacts (their bodies in L.cb). forced_body /
 // forced_h: a sphere the collision loop has just moved onto half-space
 // forced_h is in contact with it (its distance is zero in exact arithmetic;
 // rounding may leave it some ulps above the plane).
-__device__ __noinline__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, double h,
-                                         int* n_contacts, int forced_body, int forced_h) {
+__device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, double h, int* n_contacts,
+                            int forced_body, int forced_h) {
   const BucketArgs& a = V.a;
   const int lane = V.lane, nv = a.nv, nb = a.nb, Bp = a.Bp;
   const VsLayout& L = V.L;
@@ -358,14 +469,21 @@ __device__ __noinline__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, 
       }
     }
   }
-  // 4. Y = M^-1 J^T: articulated-body solves at zero velocity and gravity
-  BucketArgs am = af;
-  am.g = 0.0;
-  am.no_springs = 1;
+  // 4. Y = M^-1 J^T with M from the composite-rigid-body algorithm and its
+  // Cholesky factor (M is symmetric positive definite)
+  Strided Mm = V.col(L.mm);
+  ok = mass_matrix_cholesky(V, Mm) && ok;
   for (int r = 0; r < m; ++r) {
-    for (int k = 0; k < nv; ++k) colv[k] = J[r * nv + k];
-    ok = plain_dynamics(am, lane, q, zero, colv, out) && ok;
-    for (int k = 0; k < nv; ++k) Y[k * m + r] = out[k];
+    for (int k = 0; k < nv; ++k) {  // L y = J_r
+      double t = J[r * nv + k];
+      for (int l = 0; l < k; ++l) t -= Mm[k * nv + l] * out[l];
+      out[k] = t / Mm[k * nv + k];
+    }
+    for (int k = nv - 1; k >= 0; --k) {  // L^T y = .
+      double t = out[k];
+      for (int l = k + 1; l < nv; ++l) t -= Mm[l * nv + k] * Y[l * m + r];
+      Y[k * m + r] = t / Mm[k * nv + k];
+    }
   }
   // 5. G = J Y, g = J v_free, impulses
   for (int r = 0; r < m; ++r) {
@@ -393,7 +511,7 @@ __device__ __noinline__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, 
 // body without contacts, reaches a half-space at the new velocities
 // (integrators.rs:62-158); world frames of the current q are in scratch.
 // *hit_body / *hit_h: the sphere and half-space of that time (-1: a point).
-__device__ __noinline__ double earliest_collision(const VsLane& V, int nc, int* hit_body, int* hit_h) {
+__device__ double earliest_collision(const VsLane& V, int nc, int* hit_body, int* hit_h) {
   const BucketArgs& a = V.a;
   const int lane = V.lane, Bp = a.Bp;
   PlainCtx X{a, lane};
@@ -460,7 +578,7 @@ __device__ __noinline__ double earliest_collision(const VsLane& V, int nc, int* 
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) vs_step_kernel(BucketArgs a, double dt, int ccd) {
+__global__ void __launch_bounds__(64) vs_step_kernel(BucketArgs a, double dt, int ccd) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= a.B) return;
   const int Bp = a.Bp, nv = a.nv, nq = a.nq;
@@ -504,9 +622,12 @@ size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp
   return vs_layout(nb, nq, nv, nc, nh, ns).total * (size_t)Bp;
 }
 
+// 64-lane workgroups: one instance per lane is a long serial program, so the
+// launch spreads its waves over every compute unit rather than packing four
+// to a workgroup on a quarter of them
 int launch_vs_step(const BucketArgs& a, double dt, int ccd, void* stream) {
   if (a.B == 0) return 0;
-  dim3 block(256), grid((a.B + 255) / 256);
+  dim3 block(64), grid((a.B + 63) / 64);
   hipLaunchKernelGGL(vs_step_kernel, grid, block, 0, (hipStream_t)stream, a, dt, ccd);
   return (int)hipGetLastError();
 }
