@@ -5,13 +5,15 @@ continuous-collision tests (collision/sphere.rs, integrators.rs,
 collision/ccd/mod.rs) applied to the GPU trajectories, the random ones run as
 one batch of instances.
 
-Tolerance of the per-step check: 1e-6 relative on q and v. Both sides solve
+Tolerance of the per-step check: 1e-5 relative on q and v. Both sides solve
 the contact-impulse cone program with an interior-point method to a relative
-gap of 1e-16; where a contact slides, the impulse sits on the cone's boundary
-and the method's answer is accurate to about 1e-7 relative (the square root
-of the gap), so rounding differences between the two can move the result by
-that much. The reference's own solver (Clarabel, gap tolerance 1e-8) is
-looser still. Steps without sliding contacts agree to about 1e-12."""
+gap of 1e-16; where contacts slide, the impulse sits on the cone's boundary
+and the method's answer is only accurate to about the square root of the
+gap: the oracle's own result moves by up to 5e-6 when its input is perturbed
+by 1e-15 relative (a quadruped with sliding feet), so rounding differences
+between the two implementations move it as much. The reference's own solver
+(Clarabel, gap tolerance 1e-8) is looser still. Steps without sliding
+contacts agree to about 1e-12, and the closed-form cases to 1e-10."""
 import math
 
 import numpy as np
@@ -20,7 +22,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 VS, CCD = 3, 4
-TOL = 1e-6
+TOL = 1e-5
 
 
 def _rel(a, b):
