@@ -41,6 +41,7 @@ struct VsLane {
   int lane;
   VsLayout L;
   double* base;  // a.scratch + lane
+  double* lds;   // the workgroup's interior-point working set
   __device__ Strided col(size_t off) const { return Strided{base + off * (size_t)a.Bp, a.Bp}; }
 };
 
@@ -106,14 +107,41 @@ __device__ void nt_scaling(const double* s, const double* z, double* W, double* 
   }
 }
 
-// The impulse program for nc contacts (G, g in scratch) -> x. Returns false
+// Working arrays of the interior-point method. Up to kLdsContacts contacts
+// they live in LDS ([field][64 lanes] of the 64-lane workgroup: the Cholesky
+// factor and the solves run at LDS latency); beyond, in the lane's scratch.
+// The factor is stored packed (row r starts at r (r + 1) / 2).
+constexpr int kLdsContacts = 4;
+constexpr int kLdsM = 3 * kLdsContacts;
+enum LdsField {
+  LF_L = 0, LF_X = LF_L + kLdsM * (kLdsM + 1) / 2, LF_Z = LF_X + kLdsM, LF_S = LF_Z + kLdsM, LF_RD = LF_S + kLdsM,
+  LF_LT = LF_RD + kLdsM, LF_XB = LF_LT + kLdsM, LF_DX = LF_XB + kLdsM, LF_DZ = LF_DX + kLdsM, LF_DS = LF_DZ + kLdsM,
+  LF_XI = LF_DS + kLdsM, LF_RHS = LF_XI + kLdsM, LF_W = LF_RHS + kLdsM, LF_WI = LF_W + 9 * kLdsContacts,
+  NLF = LF_WI + 9 * kLdsContacts
+};
+struct IpmArrays {
+  Strided Lc, x, z, s, rd, lt, xb, dx, dz, ds, xi, rhs, Wc, Wic;
+};
+__device__ IpmArrays ipm_arrays(const VsLane& V, int nc, double* lds) {
+  if (nc <= kLdsContacts) {
+    auto f = [&](int field) { return Strided{lds + (size_t)field * 64 + threadIdx.x, 64}; };
+    return IpmArrays{f(LF_L), f(LF_X), f(LF_Z), f(LF_S), f(LF_RD), f(LF_LT), f(LF_XB), f(LF_DX), f(LF_DZ), f(LF_DS),
+                     f(LF_XI), f(LF_RHS), f(LF_W), f(LF_WI)};
+  }
+  const VsLayout& L = V.L;
+  return IpmArrays{V.col(L.l), V.col(L.x), V.col(L.z), V.col(L.s), V.col(L.rd), V.col(L.lt), V.col(L.xb),
+                   V.col(L.dx), V.col(L.dz), V.col(L.ds), V.col(L.xi), V.col(L.rhs), V.col(L.w), V.col(L.wi)};
+}
+
+// The impulse program for nc contacts (G, g in scratch) -> A.x. Returns false
 // when no finite iterate was produced.
-__device__ bool cone_program(const VsLane& V, int nc) {
+__device__ bool cone_program(const VsLane& V, int nc, const IpmArrays& A) {
   const int m = 3 * nc;
   const VsLayout& L = V.L;
-  Strided G = V.col(L.g), g = V.col(L.gv), Lc = V.col(L.l), x = V.col(L.x), z = V.col(L.z), s = V.col(L.s),
-          rd = V.col(L.rd), lt = V.col(L.lt), xb = V.col(L.xb), dx = V.col(L.dx), dz = V.col(L.dz), ds = V.col(L.ds),
-          xi = V.col(L.xi), rhs = V.col(L.rhs), Wc = V.col(L.w), Wic = V.col(L.wi);
+  Strided G = V.col(L.g), g = V.col(L.gv);
+  Strided Lc = A.Lc, x = A.x, z = A.z, s = A.s, rd = A.rd, lt = A.lt, xb = A.xb, dx = A.dx, dz = A.dz, ds = A.ds,
+          xi = A.xi, rhs = A.rhs, Wc = A.Wc, Wic = A.Wic;
+  auto li = [](int r, int k) { return r * (r + 1) / 2 + k; };  // packed lower triangle
   auto d = [](int k) { return k % 3 == 0 ? kConeMu : 1.0; };
   for (int c = 0; c < nc; ++c) {
     x[3 * c] = 1.0 / kConeMu; x[3 * c + 1] = 0.0; x[3 * c + 2] = 0.0;
@@ -134,13 +162,13 @@ __device__ bool cone_program(const VsLane& V, int nc) {
       }
     for (int r = 0; r < m; ++r) {
       double t = rhs[r];
-      for (int k = 0; k < r; ++k) t -= Lc[r * m + k] * dx[k];
-      dx[r] = t / Lc[r * m + r];
+      for (int k = 0; k < r; ++k) t -= Lc[li(r, k)] * dx[k];
+      dx[r] = t / Lc[li(r, r)];
     }
     for (int r = m - 1; r >= 0; --r) {
       double t = dx[r];
-      for (int k = r + 1; k < m; ++k) t -= Lc[k * m + r] * dx[k];
-      dx[r] = t / Lc[r * m + r];
+      for (int k = r + 1; k < m; ++k) t -= Lc[li(k, r)] * dx[k];
+      dx[r] = t / Lc[li(r, r)];
     }
     for (int c = 0; c < nc; ++c) {
       double a3[3];
@@ -195,25 +223,25 @@ __device__ bool cone_program(const VsLane& V, int nc) {
     }
     // K = G + D W^-2 D, Cholesky in place in Lc (lower triangle)
     for (int r = 0; r < m; ++r)
-      for (int k = 0; k <= r; ++k) Lc[r * m + k] = G[r * m + k];
+      for (int k = 0; k <= r; ++k) Lc[li(r, k)] = G[r * m + k];
     for (int c = 0; c < nc; ++c)
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j <= i; ++j) {
           double t = 0.0;
           for (int k = 0; k < 3; ++k) t += Wic[9 * c + 3 * i + k] * Wic[9 * c + 3 * k + j];
-          Lc[(3 * c + i) * m + 3 * c + j] += d(i) * t * d(j);
+          Lc[li((3 * c + i), 3 * c + j)] += d(i) * t * d(j);
         }
     bool pd = true;
     for (int j = 0; j < m && pd; ++j) {
-      double t = Lc[j * m + j];
-      for (int k = 0; k < j; ++k) t -= Lc[j * m + k] * Lc[j * m + k];
+      double t = Lc[li(j, j)];
+      for (int k = 0; k < j; ++k) t -= Lc[li(j, k)] * Lc[li(j, k)];
       if (!(t > 0.0)) { pd = false; break; }
       const double ljj = sqrt(t);
-      Lc[j * m + j] = ljj;
+      Lc[li(j, j)] = ljj;
       for (int r = j + 1; r < m; ++r) {
-        double u = Lc[r * m + j];
-        for (int k = 0; k < j; ++k) u -= Lc[r * m + k] * Lc[j * m + k];
-        Lc[r * m + j] = u / ljj;
+        double u = Lc[li(r, j)];
+        for (int k = 0; k < j; ++k) u -= Lc[li(r, k)] * Lc[li(j, k)];
+        Lc[li(r, j)] = u / ljj;
       }
     }
     if (!pd) break;
@@ -381,7 +409,7 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
   const VsLayout& L = V.L;
   PlainCtx X{a, lane};
   Strided vfree = V.col(L.vfree), zero = V.col(L.zero), colv = V.col(L.col), out = V.col(L.out), vn = V.col(L.vn);
-  Strided J = V.col(L.j), Y = V.col(L.y), G = V.col(L.g), gv = V.col(L.gv), x = V.col(L.x);
+  Strided J = V.col(L.j), Y = V.col(L.y), G = V.col(L.g), gv = V.col(L.gv);
   Strided cn = V.col(L.cn), cpn = V.col(L.cp), cb = V.col(L.cb);
   // 1. free velocity: no contact forces on this path (dynamics.rs:377-383)
   BucketArgs af = a;
@@ -497,7 +525,9 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
     for (int k = 0; k < nv; ++k) t += J[r * nv + k] * vfree[k];
     gv[r] = t;
   }
-  ok = cone_program(V, nc) && ok;
+  const IpmArrays A = ipm_arrays(V, nc, V.lds);
+  ok = cone_program(V, nc, A) && ok;
+  Strided x = A.x;
   // 6. v_next = v_free + M^-1 J^T x
   for (int k = 0; k < nv; ++k) {
     double t = vfree[k];
@@ -582,7 +612,8 @@ __global__ void __launch_bounds__(64) vs_step_kernel(BucketArgs a, double dt, in
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= a.B) return;
   const int Bp = a.Bp, nv = a.nv, nq = a.nq;
-  const VsLane V{a, lane, vs_layout(a.nb, a.nq, a.nv, a.nc, a.nh, a.ns), a.scratch + lane};
+  __shared__ double lds[NLF * 64];
+  const VsLane V{a, lane, vs_layout(a.nb, a.nq, a.nv, a.nc, a.nh, a.ns), a.scratch + lane, lds};
   Strided q{a.q + lane, Bp}, v{a.v + lane, Bp}, tau{const_cast<double*>(a.tau) + lane, Bp};
   Strided vdot{a.vdot + lane, Bp};
   Strided qc = V.col(V.L.qc), vc = V.col(V.L.vc), vn = V.col(V.L.vn), zero = V.col(V.L.zero);
