@@ -107,30 +107,28 @@ __device__ void nt_scaling(const double* s, const double* z, double* W, double* 
   }
 }
 
-// Working arrays of the interior-point method. Up to kLdsContacts contacts
-// they live in LDS ([field][64 lanes] of the 64-lane workgroup: the Cholesky
-// factor and the solves run at LDS latency); beyond, in the lane's scratch.
-// The factor is stored packed (row r starts at r (r + 1) / 2).
+// Working arrays of the interior-point method, in the lane's scratch; up to
+// kLdsContacts contacts the Cholesky factor and the Newton step, the arrays
+// its O(m^3) factorisation and O(m^2) solves read, live in LDS instead
+// ([field][64 lanes] of the 64-lane workgroup, 46 KB, within the 64 KB one
+// workgroup can address). The factor is stored packed (row r starts at
+// r (r + 1) / 2).
 constexpr int kLdsContacts = 4;
 constexpr int kLdsM = 3 * kLdsContacts;
-enum LdsField {
-  LF_L = 0, LF_X = LF_L + kLdsM * (kLdsM + 1) / 2, LF_Z = LF_X + kLdsM, LF_S = LF_Z + kLdsM, LF_RD = LF_S + kLdsM,
-  LF_LT = LF_RD + kLdsM, LF_XB = LF_LT + kLdsM, LF_DX = LF_XB + kLdsM, LF_DZ = LF_DX + kLdsM, LF_DS = LF_DZ + kLdsM,
-  LF_XI = LF_DS + kLdsM, LF_RHS = LF_XI + kLdsM, LF_W = LF_RHS + kLdsM, LF_WI = LF_W + 9 * kLdsContacts,
-  NLF = LF_WI + 9 * kLdsContacts
-};
+enum LdsField { LF_L = 0, LF_DX = LF_L + kLdsM * (kLdsM + 1) / 2, NLF = LF_DX + kLdsM };
+static_assert(NLF * 64 * sizeof(double) <= 65536, "LDS working set above 64 KB");
 struct IpmArrays {
   Strided Lc, x, z, s, rd, lt, xb, dx, dz, ds, xi, rhs, Wc, Wic;
 };
 __device__ IpmArrays ipm_arrays(const VsLane& V, int nc, double* lds) {
-  if (nc <= kLdsContacts) {
-    auto f = [&](int field) { return Strided{lds + (size_t)field * 64 + threadIdx.x, 64}; };
-    return IpmArrays{f(LF_L), f(LF_X), f(LF_Z), f(LF_S), f(LF_RD), f(LF_LT), f(LF_XB), f(LF_DX), f(LF_DZ), f(LF_DS),
-                     f(LF_XI), f(LF_RHS), f(LF_W), f(LF_WI)};
-  }
   const VsLayout& L = V.L;
-  return IpmArrays{V.col(L.l), V.col(L.x), V.col(L.z), V.col(L.s), V.col(L.rd), V.col(L.lt), V.col(L.xb),
-                   V.col(L.dx), V.col(L.dz), V.col(L.ds), V.col(L.xi), V.col(L.rhs), V.col(L.w), V.col(L.wi)};
+  IpmArrays A{V.col(L.l), V.col(L.x), V.col(L.z), V.col(L.s), V.col(L.rd), V.col(L.lt), V.col(L.xb),
+              V.col(L.dx), V.col(L.dz), V.col(L.ds), V.col(L.xi), V.col(L.rhs), V.col(L.w), V.col(L.wi)};
+  if (nc <= kLdsContacts) {
+    A.Lc = Strided{lds + (size_t)LF_L * 64 + threadIdx.x, 64};
+    A.dx = Strided{lds + (size_t)LF_DX * 64 + threadIdx.x, 64};
+  }
+  return A;
 }
 
 // The impulse program for nc contacts (G, g in scratch) -> A.x. Returns false
