@@ -41,7 +41,6 @@ struct VsLane {
   int lane;
   VsLayout L;
   double* base;  // a.scratch + lane
-  double* lds;   // the workgroup's interior-point working set
   __device__ Strided col(size_t off) const { return Strided{base + off * (size_t)a.Bp, a.Bp}; }
 };
 
@@ -110,35 +109,30 @@ __device__ void nt_scaling(const double* s, const double* z, double* W, double* 
 // Working arrays of the interior-point method, in the lane's scratch; up to
 // kLdsContacts contacts the Cholesky factor and the Newton step, the arrays
 // its O(m^3) factorisation and O(m^2) solves read, live in LDS instead
-// ([field][64 lanes] of the 64-lane workgroup, 46 KB, within the 64 KB one
-// workgroup can address). The factor is stored packed (row r starts at
-// r (r + 1) / 2).
+// ([field][64 lanes] of the 64-lane workgroup, 46 KB). The factor is stored
+// packed (row r starts at r (r + 1) / 2).
+// The LDS arrays are reached through the __shared__ symbol itself, never
+// through a pointer that could also hold a global address.
 constexpr int kLdsContacts = 4;
 constexpr int kLdsM = 3 * kLdsContacts;
 enum LdsField { LF_L = 0, LF_DX = LF_L + kLdsM * (kLdsM + 1) / 2, NLF = LF_DX + kLdsM };
 static_assert(NLF * 64 * sizeof(double) <= 65536, "LDS working set above 64 KB");
-struct IpmArrays {
-  Strided Lc, x, z, s, rd, lt, xb, dx, dz, ds, xi, rhs, Wc, Wic;
+__shared__ double vs_lds[NLF * 64];
+struct LdsCol {  // field `off` of this lane's LDS column
+  int off;
+  __device__ double& operator[](int k) const { return vs_lds[(off + k) * 64 + (int)threadIdx.x]; }
 };
-__device__ IpmArrays ipm_arrays(const VsLane& V, int nc, double* lds) {
-  const VsLayout& L = V.L;
-  IpmArrays A{V.col(L.l), V.col(L.x), V.col(L.z), V.col(L.s), V.col(L.rd), V.col(L.lt), V.col(L.xb),
-              V.col(L.dx), V.col(L.dz), V.col(L.ds), V.col(L.xi), V.col(L.rhs), V.col(L.w), V.col(L.wi)};
-  if (nc <= kLdsContacts) {
-    A.Lc = Strided{lds + (size_t)LF_L * 64 + threadIdx.x, 64};
-    A.dx = Strided{lds + (size_t)LF_DX * 64 + threadIdx.x, 64};
-  }
-  return A;
-}
 
-// The impulse program for nc contacts (G, g in scratch) -> A.x. Returns false
-// when no finite iterate was produced.
-__device__ bool cone_program(const VsLane& V, int nc, const IpmArrays& A) {
+// The impulse program for nc contacts (G, g in scratch) -> x. Returns false
+// when no finite iterate was produced. Lc / dx: the factor and the Newton
+// step, in LDS (LdsCol) or in the lane's scratch (Strided).
+template <class Fac>
+__device__ bool cone_program(const VsLane& V, int nc, Fac Lc, Fac dx) {
   const int m = 3 * nc;
   const VsLayout& L = V.L;
   Strided G = V.col(L.g), g = V.col(L.gv);
-  Strided Lc = A.Lc, x = A.x, z = A.z, s = A.s, rd = A.rd, lt = A.lt, xb = A.xb, dx = A.dx, dz = A.dz, ds = A.ds,
-          xi = A.xi, rhs = A.rhs, Wc = A.Wc, Wic = A.Wic;
+  Strided x = V.col(L.x), z = V.col(L.z), s = V.col(L.s), rd = V.col(L.rd), lt = V.col(L.lt), xb = V.col(L.xb),
+          dz = V.col(L.dz), ds = V.col(L.ds), xi = V.col(L.xi), rhs = V.col(L.rhs), Wc = V.col(L.w), Wic = V.col(L.wi);
   auto li = [](int r, int k) { return r * (r + 1) / 2 + k; };  // packed lower triangle
   auto d = [](int k) { return k % 3 == 0 ? kConeMu : 1.0; };
   for (int c = 0; c < nc; ++c) {
@@ -523,9 +517,11 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
     for (int k = 0; k < nv; ++k) t += J[r * nv + k] * vfree[k];
     gv[r] = t;
   }
-  const IpmArrays A = ipm_arrays(V, nc, V.lds);
-  ok = cone_program(V, nc, A) && ok;
-  Strided x = A.x;
+  const bool solved = nc <= kLdsContacts
+                          ? cone_program(V, nc, LdsCol{LF_L}, LdsCol{LF_DX})
+                          : cone_program(V, nc, V.col(L.l), V.col(L.dx));
+  ok = solved && ok;
+  Strided x = V.col(L.x);
   // 6. v_next = v_free + M^-1 J^T x
   for (int k = 0; k < nv; ++k) {
     double t = vfree[k];
@@ -610,8 +606,7 @@ __global__ void __launch_bounds__(64) vs_step_kernel(BucketArgs a, double dt, in
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= a.B) return;
   const int Bp = a.Bp, nv = a.nv, nq = a.nq;
-  __shared__ double lds[NLF * 64];
-  const VsLane V{a, lane, vs_layout(a.nb, a.nq, a.nv, a.nc, a.nh, a.ns), a.scratch + lane, lds};
+  const VsLane V{a, lane, vs_layout(a.nb, a.nq, a.nv, a.nc, a.nh, a.ns), a.scratch + lane};
   Strided q{a.q + lane, Bp}, v{a.v + lane, Bp}, tau{const_cast<double*>(a.tau) + lane, Bp};
   Strided vdot{a.vdot + lane, Bp};
   Strided qc = V.col(V.L.qc), vc = V.col(V.L.vc), vn = V.col(V.L.vn), zero = V.col(V.L.zero);
