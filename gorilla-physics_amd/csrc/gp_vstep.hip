@@ -106,34 +106,14 @@ __device__ void nt_scaling(const double* s, const double* z, double* W, double* 
   }
 }
 
-// Working arrays of the interior-point method, in the lane's scratch; up to
-// kLdsContacts contacts the Cholesky factor and the Newton step, the arrays
-// its O(m^3) factorisation and O(m^2) solves read, live in LDS instead
-// ([field][64 lanes] of the 64-lane workgroup, 46 KB). The factor is stored
-// packed (row r starts at r (r + 1) / 2).
-// The LDS arrays are reached through the __shared__ symbol itself, never
-// through a pointer that could also hold a global address.
-constexpr int kLdsContacts = 4;
-constexpr int kLdsM = 3 * kLdsContacts;
-enum LdsField { LF_L = 0, LF_DX = LF_L + kLdsM * (kLdsM + 1) / 2, NLF = LF_DX + kLdsM };
-static_assert(NLF * 64 * sizeof(double) <= 65536, "LDS working set above 64 KB");
-__shared__ double vs_lds[NLF * 64];
-struct LdsCol {  // field `off` of this lane's LDS column
-  int off;
-  __device__ double& operator[](int k) const { return vs_lds[(off + k) * 64 + (int)threadIdx.x]; }
-};
-
 // The impulse program for nc contacts (G, g in scratch) -> x. Returns false
-// when no finite iterate was produced. Lc / dx: the factor and the Newton
-// step, in LDS (LdsCol) or in the lane's scratch (Strided).
-template <class Fac>
-__device__ bool cone_program(const VsLane& V, int nc, Fac Lc, Fac dx) {
+// when no finite iterate was produced.
+__device__ bool cone_program(const VsLane& V, int nc) {
   const int m = 3 * nc;
   const VsLayout& L = V.L;
-  Strided G = V.col(L.g), g = V.col(L.gv);
-  Strided x = V.col(L.x), z = V.col(L.z), s = V.col(L.s), rd = V.col(L.rd), lt = V.col(L.lt), xb = V.col(L.xb),
-          dz = V.col(L.dz), ds = V.col(L.ds), xi = V.col(L.xi), rhs = V.col(L.rhs), Wc = V.col(L.w), Wic = V.col(L.wi);
-  auto li = [](int r, int k) { return r * (r + 1) / 2 + k; };  // packed lower triangle
+  Strided G = V.col(L.g), g = V.col(L.gv), Lc = V.col(L.l), x = V.col(L.x), z = V.col(L.z), s = V.col(L.s),
+          rd = V.col(L.rd), lt = V.col(L.lt), xb = V.col(L.xb), dx = V.col(L.dx), dz = V.col(L.dz), ds = V.col(L.ds),
+          xi = V.col(L.xi), rhs = V.col(L.rhs), Wc = V.col(L.w), Wic = V.col(L.wi);
   auto d = [](int k) { return k % 3 == 0 ? kConeMu : 1.0; };
   for (int c = 0; c < nc; ++c) {
     x[3 * c] = 1.0 / kConeMu; x[3 * c + 1] = 0.0; x[3 * c + 2] = 0.0;
@@ -154,13 +134,13 @@ __device__ bool cone_program(const VsLane& V, int nc, Fac Lc, Fac dx) {
       }
     for (int r = 0; r < m; ++r) {
       double t = rhs[r];
-      for (int k = 0; k < r; ++k) t -= Lc[li(r, k)] * dx[k];
-      dx[r] = t / Lc[li(r, r)];
+      for (int k = 0; k < r; ++k) t -= Lc[r * m + k] * dx[k];
+      dx[r] = t / Lc[r * m + r];
     }
     for (int r = m - 1; r >= 0; --r) {
       double t = dx[r];
-      for (int k = r + 1; k < m; ++k) t -= Lc[li(k, r)] * dx[k];
-      dx[r] = t / Lc[li(r, r)];
+      for (int k = r + 1; k < m; ++k) t -= Lc[k * m + r] * dx[k];
+      dx[r] = t / Lc[r * m + r];
     }
     for (int c = 0; c < nc; ++c) {
       double a3[3];
@@ -215,25 +195,25 @@ __device__ bool cone_program(const VsLane& V, int nc, Fac Lc, Fac dx) {
     }
     // K = G + D W^-2 D, Cholesky in place in Lc (lower triangle)
     for (int r = 0; r < m; ++r)
-      for (int k = 0; k <= r; ++k) Lc[li(r, k)] = G[r * m + k];
+      for (int k = 0; k <= r; ++k) Lc[r * m + k] = G[r * m + k];
     for (int c = 0; c < nc; ++c)
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j <= i; ++j) {
           double t = 0.0;
           for (int k = 0; k < 3; ++k) t += Wic[9 * c + 3 * i + k] * Wic[9 * c + 3 * k + j];
-          Lc[li((3 * c + i), 3 * c + j)] += d(i) * t * d(j);
+          Lc[(3 * c + i) * m + 3 * c + j] += d(i) * t * d(j);
         }
     bool pd = true;
     for (int j = 0; j < m && pd; ++j) {
-      double t = Lc[li(j, j)];
-      for (int k = 0; k < j; ++k) t -= Lc[li(j, k)] * Lc[li(j, k)];
+      double t = Lc[j * m + j];
+      for (int k = 0; k < j; ++k) t -= Lc[j * m + k] * Lc[j * m + k];
       if (!(t > 0.0)) { pd = false; break; }
       const double ljj = sqrt(t);
-      Lc[li(j, j)] = ljj;
+      Lc[j * m + j] = ljj;
       for (int r = j + 1; r < m; ++r) {
-        double u = Lc[li(r, j)];
-        for (int k = 0; k < j; ++k) u -= Lc[li(r, k)] * Lc[li(j, k)];
-        Lc[li(r, j)] = u / ljj;
+        double u = Lc[r * m + j];
+        for (int k = 0; k < j; ++k) u -= Lc[r * m + k] * Lc[j * m + k];
+        Lc[r * m + j] = u / ljj;
       }
     }
     if (!pd) break;
@@ -401,7 +381,7 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
   const VsLayout& L = V.L;
   PlainCtx X{a, lane};
   Strided vfree = V.col(L.vfree), zero = V.col(L.zero), colv = V.col(L.col), out = V.col(L.out), vn = V.col(L.vn);
-  Strided J = V.col(L.j), Y = V.col(L.y), G = V.col(L.g), gv = V.col(L.gv);
+  Strided J = V.col(L.j), Y = V.col(L.y), G = V.col(L.g), gv = V.col(L.gv), x = V.col(L.x);
   Strided cn = V.col(L.cn), cpn = V.col(L.cp), cb = V.col(L.cb);
   // 1. free velocity: no contact forces on this path (dynamics.rs:377-383)
   BucketArgs af = a;
@@ -517,11 +497,7 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
     for (int k = 0; k < nv; ++k) t += J[r * nv + k] * vfree[k];
     gv[r] = t;
   }
-  const bool solved = nc <= kLdsContacts
-                          ? cone_program(V, nc, LdsCol{LF_L}, LdsCol{LF_DX})
-                          : cone_program(V, nc, V.col(L.l), V.col(L.dx));
-  ok = solved && ok;
-  Strided x = V.col(L.x);
+  ok = cone_program(V, nc) && ok;
   // 6. v_next = v_free + M^-1 J^T x
   for (int k = 0; k < nv; ++k) {
     double t = vfree[k];
