@@ -107,14 +107,8 @@ __device__ void nt_scaling(const double* s, const double* z, double* W, double* 
 }
 
 // The impulse program for nc contacts (G, g in scratch) -> x. Returns false
-// when no finite iterate was produced. NC > 0 fixes the contact count at
-// compile time: the loops over the m = 3 NC rows then unroll, so the loads of
-// a factorisation or solve step are issued together instead of one after the
-// other (the working set is in scratch, and its latency is what bounds this
-// kernel); NC = 0 takes the count at run time.
-template <int NC>
-__device__ bool cone_program(const VsLane& V, int nc_rt) {
-  const int nc = NC > 0 ? NC : nc_rt;
+// when no finite iterate was produced.
+__device__ bool cone_program(const VsLane& V, int nc) {
   const int m = 3 * nc;
   const VsLayout& L = V.L;
   Strided G = V.col(L.g), g = V.col(L.gv), Lc = V.col(L.l), x = V.col(L.x), z = V.col(L.z), s = V.col(L.s),
@@ -503,15 +497,7 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
     for (int k = 0; k < nv; ++k) t += J[r * nv + k] * vfree[k];
     gv[r] = t;
   }
-  bool solved;
-  switch (nc) {  // resting cubes touch on four corners, spheres on one point
-    case 1: solved = cone_program<1>(V, nc); break;
-    case 2: solved = cone_program<2>(V, nc); break;
-    case 3: solved = cone_program<3>(V, nc); break;
-    case 4: solved = cone_program<4>(V, nc); break;
-    default: solved = cone_program<0>(V, nc); break;
-  }
-  ok = solved && ok;
+  ok = cone_program(V, nc) && ok;
   // 6. v_next = v_free + M^-1 J^T x
   for (int k = 0; k < nv; ++k) {
     double t = vfree[k];
