@@ -157,6 +157,7 @@ SIGNATURES = {
     "gp_batch_get_state_dev": (C.c_int, [_P, C.c_void_p, C.c_void_p, _P]),
     "gp_batch_set_tau_dev": (C.c_int, [_P, C.c_void_p, _P]),
     "gp_batch_get_vdot_dev": (C.c_int, [_P, C.c_void_p, _P]),
+    "gp_batch_energy": (C.c_int, [_P, _D, _D]),
     "gp_batch_set_spring_contact": (C.c_int, [_P, C.c_int64, C.c_int32, _D, C.c_double]),
     "gp_batch_get_spring_contact": (C.c_int, [_P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                               _D, _D, _D]),

@@ -156,6 +156,13 @@ class Batch:
         k = min(n.value, max_records)
         return qs[:k], vs[:k]
 
+    def energy(self):
+        """(kinetic, gravitational) energy of every instance at the current
+        state (gp_batch_energy, mechanism.rs:334-362)."""
+        ke, pe = np.empty(self.n_instances), np.empty(self.n_instances)
+        abi.check(self.lib.gp_batch_energy(self.handle, _dp(ke), _dp(pe)), self.lib)
+        return ke, pe
+
     def faults(self):
         f = np.zeros(self.n_instances, dtype=np.uint8)
         abi.check(self.lib.gp_batch_faults(self.handle, f.ctypes.data_as(C.POINTER(C.c_uint8))), self.lib)

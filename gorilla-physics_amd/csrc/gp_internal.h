@@ -172,6 +172,8 @@ size_t plain_scratch_doubles(int nb, int nq, int nv, int Bp);
 // velocity stepping (ccd = 1: with continuous collision detection), FP64, one
 // step per launch; scratch as vs_layout
 int launch_vs_step(const BucketArgs& a, double dt, int ccd, void* stream);
+// kinetic and gravitational energy per lane (ke[lane], pe[lane])
+int launch_energy(const BucketArgs& a, double* ke, double* pe, void* stream);
 size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp);
 // rows x B values between a bucket's SoA rows and a flat instance-major array
 // (lane l's values start at flat + flat_off[l]); to_soa = 1: flat -> SoA.

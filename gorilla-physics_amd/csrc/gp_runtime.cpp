@@ -995,6 +995,32 @@ int gp_simulate(gp_batch* bt, double final_time, double dt, gp_control_fn contro
   return GP_OK;
 }
 
+int gp_batch_energy(gp_batch* bt, double* kinetic, double* gravitational) {
+  if (!bt) return fail(GP_ERR_INVALID, "null batch");
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  for (Bucket& b : bt->buckets) {
+    int st = bucket_ensure_scratch(b);
+    if (st) return st;
+    double* d = nullptr;
+    HIP_TRY(hipMalloc((void**)&d, sizeof(double) * 2 * b.Bp));
+    BucketArgs a = bucket_args(b, 1, nullptr);
+    int e = launch_energy(a, d, d + b.Bp, bt->stream);
+    std::vector<double> h(2 * (size_t)b.Bp);
+    hipError_t ce = e ? hipSuccess : hipMemcpyAsync(h.data(), d, sizeof(double) * 2 * b.Bp, hipMemcpyDeviceToHost,
+                                                    bt->stream);
+    hipError_t se = hipStreamSynchronize(bt->stream);
+    (void)hipFree(d);
+    if (e) return fail(GP_ERR_DEVICE, "energy launch: %s", hipGetErrorString((hipError_t)e));
+    if (ce != hipSuccess || se != hipSuccess)
+      return fail(GP_ERR_DEVICE, "energy copy: %s", hipGetErrorString(ce != hipSuccess ? ce : se));
+    for (int l = 0; l < b.B; ++l) {
+      if (kinetic) kinetic[b.inst[l]] = h[l];
+      if (gravitational) gravitational[b.inst[l]] = h[b.Bp + l];
+    }
+  }
+  return GP_OK;
+}
+
 int gp_batch_faults(gp_batch* bt, uint8_t* faults) {
   if (!bt || !faults) return fail(GP_ERR_INVALID, "null argument");
   HIP_TRY(hipStreamSynchronize(bt->stream));

@@ -570,6 +570,16 @@ class MechanismState:
     def set_joint_v(self, jointid, v):
         self.v[jointid - 1] = v
 
+    def kinetic_energy(self):  # mechanism.rs:334
+        b = self._device_batch()
+        b.set_state(self.flat_q(), self.flat_v())
+        return float(b.energy()[0][0])
+
+    def gravitational_energy(self):  # mechanism.rs:352 (height of each body frame's origin)
+        b = self._device_batch()
+        b.set_state(self.flat_q(), self.flat_v())
+        return float(b.energy()[1][0])
+
     def poses(self):
         """World pose of every body at the current q (mechanism.rs:403-418)."""
         world, out = [], []

@@ -273,6 +273,13 @@ int gp_simulate(gp_batch* batch, double final_time, double dt, gp_control_fn con
                 void* user, int32_t integrator, double* qs, double* vs, int64_t max_records,
                 int64_t* n_records);
 
+/* MechanismState::kinetic_energy and gravitational_energy (mechanism.rs:334-362)
+ * of every instance at the batch's current state: kinetic[i] = sum over
+ * bodies of 1/2 (w.J w + v.(m v + 2 w x c)) in world coordinates
+ * (inertia.rs:182-202); gravitational[i] = sum of m g z of each body frame's
+ * origin, as the reference computes it. Either pointer may be NULL. FP64. */
+int gp_batch_energy(gp_batch* batch, double* kinetic, double* gravitational);
+
 /* Per-instance fault flags (1 = a non-finite q/v/vdot, a non-positive D or a
  * spring contact pulling into its half-space was seen since the last clear).
  * No reference counterpart: the reference panics (dynamics.rs:267-274,

@@ -246,6 +246,17 @@ int oracle_energy(const gp_model_desc* d, const double* q, const double* v, doub
   return 0;
 }
 
+// MechanismState::gravitational_energy (mechanism.rs:352-362): m g z of each
+// body frame's origin (not of the centre of mass).
+int oracle_gravitational_energy(const gp_model_desc* d, const double* q, double* pe) {
+  Model<double> M = load_model<double>(*d);
+  std::vector<Iso<double>> b2r = bodies_to_root(M, q);
+  double P = 0.0;
+  for (int i = 0; i < M.n; ++i) P += M.I[i].m * M.g * b2r[i].t.z;
+  *pe = P;
+  return 0;
+}
+
 // signed distance (p - h.point) . n of every (contact point, half-space) pair,
 // contact-major in the order of desc->contacts (halfspace.rs:46)
 int oracle_contact_distances(const gp_model_desc* d, const double* q, double* dist) {

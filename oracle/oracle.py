@@ -32,6 +32,7 @@ class Oracle:
         lib.oracle_step.argtypes = [P, C.c_int, C.c_int, _D, _D, _D, C.c_double, _D, _D, _D]
         lib.oracle_step_spring.argtypes = [P, _D, _D, _D, C.c_double, _D, _D, _D, _D]
         lib.oracle_vs_step.argtypes = [P, C.c_int, _D, _D, _D, C.c_double, _D, _D, C.POINTER(C.c_int)]
+        lib.oracle_gravitational_energy.argtypes = [P, _D, _D]
         lib.oracle_cone_program.argtypes = [C.c_int, _D, _D, _D, C.POINTER(C.c_int)]
         lib.oracle_rollout.argtypes = [C.POINTER(P), C.c_int64, _D, _D, _D, C.c_double, C.c_int, C.c_int,
                                        C.c_int64, C.c_int, _D]
@@ -157,6 +158,13 @@ class Oracle:
         q = np.ascontiguousarray(q, float); v = np.ascontiguousarray(v, float)
         self.lib.oracle_energy(self._desc(d), _dp(q), _dp(v), C.byref(ke), C.byref(pe))
         return ke.value, pe.value
+
+    def gravitational_energy(self, d, q):
+        """sum m g z of every body frame's origin (mechanism.rs:352-362)."""
+        pe = C.c_double()
+        q = np.ascontiguousarray(q, float)
+        self.lib.oracle_gravitational_energy(self._desc(d), _dp(q), C.byref(pe))
+        return pe.value
 
     def contact_distances(self, d, q):
         q = np.ascontiguousarray(q, float)

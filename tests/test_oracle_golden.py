@@ -452,3 +452,18 @@ def test_spring_contact_vertical_bounce_closed_form(gp, oracle):
     assert tops and abs(tops[0]) < 1e-4
     s = at_top[0]
     assert s[0] == 0.0 and np.allclose(s[5:8], [0, 0, -1], atol=1e-12) and s[8] == l_rest
+
+
+def test_gravitational_energy_uses_frame_origins(gp, oracle):
+    """gravitational_energy (mechanism.rs:352-362) sums m g z of each body
+    frame's origin: a horizontal rod whose centre of mass is l/2 off its
+    joint has the energy of its joint height, not of its centre of mass."""
+    m, l, h = 2.0, 1.0, 0.7
+    st = rod_model(gp, m, l, gp.Isometry3.translation_xyz(0.0, 0.0, h), [0, 1, 0])
+    d = st.model_desc()
+    assert abs(oracle.gravitational_energy(d, [0.0]) - m * G * h) < 1e-12
+    # rotated by -90 degrees about y the rod points up: same frame origin, so
+    # the same value, while the centre-of-mass potential rises by m g l / 2
+    assert abs(oracle.gravitational_energy(d, [-math.pi / 2]) - m * G * h) < 1e-12
+    _, pe_com = oracle.energy(d, [-math.pi / 2], [0.0])
+    assert abs(pe_com - m * G * (h + l / 2)) < 1e-12
