@@ -3,7 +3,7 @@
 Default workload: 65,536 serial 7-DOF revolute arms (random links, axes,
 masses, inertias, constant torques), dt = 1e-3, semi-implicit Euler, mixed
 precision (FP32 sweeps, FP64 D solve and q/v accumulation). One "step" is one
-dt of every instance. Prints one JSON line (the driver's contract) with
+dt of every instance. Prints one JSON line with
 env-steps/s, the dominant kernel's VALU roofline (algorithmic flops from the
 ABA op count, per-launch duration from HIP events on the batch's stream),
 the plain-FP64 and fast-FP64 kernels measured on the same inputs, and the
@@ -264,7 +264,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: instances are independent, so several "
-                                   "GPUs would each run their own share (see DESIGN.md); pass --gpus 1"}))
+                                   "GPUs would each run their own share of the instances; pass --gpus 1"}))
         sys.exit(2)
     load_package()
     from gorilla_physics_amd import abi, batch
