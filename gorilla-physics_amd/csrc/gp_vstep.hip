@@ -107,9 +107,7 @@ __device__ void nt_scaling(const double* s, const double* z, double* W, double* 
 }
 
 // The impulse program for nc contacts (G, g in scratch) -> x. Returns false
-// when no finite iterate was produced. The inner loops are unrolled four-fold
-// so that a row's loads from scratch are issued together: this kernel waits
-// on the latency of its working set, not on arithmetic.
+// when no finite iterate was produced.
 __device__ bool cone_program(const VsLane& V, int nc) {
   const int m = 3 * nc;
   const VsLayout& L = V.L;
@@ -136,13 +134,11 @@ __device__ bool cone_program(const VsLane& V, int nc) {
       }
     for (int r = 0; r < m; ++r) {
       double t = rhs[r];
-      #pragma unroll 4
       for (int k = 0; k < r; ++k) t -= Lc[r * m + k] * dx[k];
       dx[r] = t / Lc[r * m + r];
     }
     for (int r = m - 1; r >= 0; --r) {
       double t = dx[r];
-      #pragma unroll 4
       for (int k = r + 1; k < m; ++k) t -= Lc[k * m + r] * dx[k];
       dx[r] = t / Lc[r * m + r];
     }
@@ -173,7 +169,6 @@ __device__ bool cone_program(const VsLane& V, int nc) {
     for (int k = 0; k < m; ++k) s[k] = d(k) * x[k];
     for (int r = 0; r < m; ++r) {
       double t = g[r] - d(r) * z[r];
-      #pragma unroll 4
       for (int k = 0; k < m; ++k) t += G[r * m + k] * x[k];
       rd[r] = t;
       rmax = fmax(rmax, fabs(t));
@@ -200,7 +195,6 @@ __device__ bool cone_program(const VsLane& V, int nc) {
     }
     // K = G + D W^-2 D, Cholesky in place in Lc (lower triangle)
     for (int r = 0; r < m; ++r)
-      #pragma unroll 4
       for (int k = 0; k <= r; ++k) Lc[r * m + k] = G[r * m + k];
     for (int c = 0; c < nc; ++c)
       for (int i = 0; i < 3; ++i)
@@ -212,14 +206,12 @@ __device__ bool cone_program(const VsLane& V, int nc) {
     bool pd = true;
     for (int j = 0; j < m && pd; ++j) {
       double t = Lc[j * m + j];
-      #pragma unroll 4
       for (int k = 0; k < j; ++k) t -= Lc[j * m + k] * Lc[j * m + k];
       if (!(t > 0.0)) { pd = false; break; }
       const double ljj = sqrt(t);
       Lc[j * m + j] = ljj;
       for (int r = j + 1; r < m; ++r) {
         double u = Lc[r * m + j];
-        #pragma unroll 4
         for (int k = 0; k < j; ++k) u -= Lc[r * m + k] * Lc[j * m + k];
         Lc[r * m + j] = u / ljj;
       }
