@@ -272,3 +272,30 @@ def test_velocity_stepping_rejects_mixed_plain(gp):
     with pytest.raises(gp.abi.GPError) as e:
         b.step(1e-3, VS, 1)
     assert e.value.code == gp.abi.GP_ERR_UNSUPPORTED
+
+
+def test_reference_tests_through_the_mechanism_mirror(gp):
+    """sphere_hit_ground (collision/sphere.rs:58) and ccd_sphere_ground
+    (collision/ccd/mod.rs:38) written as the reference writes them, against
+    the Python mirror of MechanismState / step / poses."""
+    from gorilla_physics_amd import builders
+    print("case reference sphere tests (mechanism mirror)", flush=True)
+    state = builders.build_sphere(1.0, 1.0)
+    state.update([gp.Pose(gp.UnitQuaternion.identity(), [0.0, 0.0, 1.5])], [gp.SpatialVector()])
+    state.add_halfspace(gp.HalfSpace.new([0.0, 0.0, 1.0], 0.0))
+    for _ in range(int(2.0 / 1e-3)):
+        gp.step(state, 1e-3, [], gp.Integrator.VelocityStepping)
+    assert np.abs(np.asarray(state.poses()[0].translation) - [0, 0, 1.0]).max() < 1e-3
+
+    radius = 0.1
+    state = builders.build_sphere(1.0, radius)
+    state.add_halfspace(gp.HalfSpace.new([0.0, 0.0, 1.0], 0.0))
+    state.set_joint_q(1, gp.Pose(gp.UnitQuaternion.identity(), [0.0, 0.0, 10 * radius]))
+    state.set_joint_v(1, gp.SpatialVector([0.0, 0.0, 0.0], [0.0, 0.0, -5.0]))
+    max_penetration = 0.0
+    for _ in range(int(1.0 / (1.0 / 60.0))):
+        gp.step(state, 1.0 / 60.0, [], gp.Integrator.CCDVelocityStepping)
+        z = state.poses()[0].translation[2]
+        max_penetration = max(max_penetration, max(radius - z, 0.0))
+    assert abs(state.poses()[0].translation[2] - radius) < 1e-8
+    assert max_penetration < 1e-8
