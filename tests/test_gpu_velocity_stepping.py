@@ -299,3 +299,19 @@ def test_reference_tests_through_the_mechanism_mirror(gp):
         max_penetration = max(max_penetration, max(radius - z, 0.0))
     assert abs(state.poses()[0].translation[2] - radius) < 1e-8
     assert max_penetration < 1e-8
+
+
+def test_interface_simulator_drops_a_cube(gp):
+    """The browser client's InterfaceSimulator (interface/mod.rs:76-118)
+    through this library: a cube dropped on the ground with CCD velocity
+    stepping comes to rest on a face."""
+    from gorilla_physics_amd import builders, interface
+    print("case interface simulator cube", flush=True)
+    st = builders.build_cube(1.0, 0.5)
+    st.update([gp.Pose(gp.UnitQuaternion.from_axis_angle([1.0, 0, 0], 0.3), [0.0, 0.0, 1.0])], [gp.SpatialVector()])
+    sim = interface.InterfaceSimulator(st)
+    sim.addHalfSpace([0, 0, 1.0], 0.0)
+    for _ in range(120):
+        q = sim.step(1.0 / 60.0, [])
+    assert q.shape == (7,) and abs(q[6] - 0.25) < 1e-6
+    assert abs(sim.contact_positions().reshape(-1, 3)[:, 2].min()) < 1e-6

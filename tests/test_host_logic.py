@@ -90,3 +90,22 @@ def test_mixed_workload_shape(gp):
     assert set(nb) - {13} <= set(range(1, 13))
     assert sum(d.nq for d in w["models"]) == len(w["q"])
     assert sum(d.nv for d in w["models"]) == len(w["v"]) == len(w["tau"])
+
+
+def test_interface_poses_and_contact_positions(gp):
+    """The browser-client calls (interface/mod.rs:172-220): poses as
+    [qx, qy, qz, qw, tx, ty, tz] per body, contact positions in the world."""
+    from gorilla_physics_amd import builders, interface
+    st = builders.build_cube(1.0, 2.0)
+    rot = gp.UnitQuaternion.from_axis_angle([0, 0, 1.0], math.pi / 2)
+    st.update([gp.Pose(rot, [1.0, 2.0, 3.0])], [gp.SpatialVector()])
+    sim = interface.InterfaceSimulator(st)
+    p = sim.poses()
+    assert p.shape == (7,) and np.allclose(p[:4], rot.coords()) and np.allclose(p[4:], [1, 2, 3])
+    c = sim.contact_positions().reshape(-1, 3)
+    assert c.shape == (8, 3)
+    # corners at (+-1, +-1, +-1) rotated 90 degrees about z, then moved
+    assert np.allclose(sorted(map(tuple, np.round(c - [1, 2, 3], 12))),
+                       sorted((x, y, z) for x in (-1.0, 1.0) for y in (-1.0, 1.0) for z in (-1.0, 1.0)))
+    sim.addHalfSpace([0, 0, 2.0], 0.5)
+    assert np.allclose(st.halfspaces[0].normal, [0, 0, 1]) and np.allclose(st.halfspaces[0].point, [0, 0, 0.5])
