@@ -61,8 +61,11 @@ struct PlainCtx {
 };
 
 // Articulated-body forward dynamics of one lane; q, v, tau, vdot are the
-// lane's columns of [k][Bp] arrays. Returns false on a non-positive D.
-__device__ inline bool plain_dynamics(const BucketArgs& a, int lane, Strided q, Strided v, Strided tau, Strided vdot) {
+// lane's columns of [k][Bp] arrays. contacts = false leaves out the contact
+// and spring-contact forces (the velocity-stepping path). Returns false on a
+// non-positive D.
+__device__ inline bool plain_dynamics(const BucketArgs& a, int lane, Strided q, Strided v, Strided tau, Strided vdot,
+                                      bool contacts = true) {
   PlainCtx X{a, lane};
   const int nb = a.nb, Bp = a.Bp;
   bool ok = true;
@@ -105,7 +108,7 @@ __device__ inline bool plain_dynamics(const BucketArgs& a, int lane, Strided q, 
     v3<double> cm{X.bpar(BF_C, i), X.bpar(BF_C + 1, i), X.bpar(BF_C + 2, i)};
     ai6<double> I = rigid(J, cm, X.bpar(BF_M, i));
     sv<double> f = crf(vel, mul(I, vel));
-    if (a.nc > 0 || a.nsp > 0) {
+    if (contacts && (a.nc > 0 || a.nsp > 0)) {
       m3<double> Rw = Rm;
       v3<double> tw = p;
       if (pr >= 0) {

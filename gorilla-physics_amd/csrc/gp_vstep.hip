@@ -384,10 +384,7 @@ __device__ bool vs_dynamics(const VsLane& V, Strided q, Strided v, Strided tau, 
   Strided J = V.col(L.j), Y = V.col(L.y), G = V.col(L.g), gv = V.col(L.gv), x = V.col(L.x);
   Strided cn = V.col(L.cn), cpn = V.col(L.cp), cb = V.col(L.cb);
   // 1. free velocity: no contact forces on this path (dynamics.rs:377-383)
-  BucketArgs af = a;
-  af.nc = 0;
-  af.nsp = 0;
-  bool ok = plain_dynamics(af, lane, q, v, tau, out);
+  bool ok = plain_dynamics(a, lane, q, v, tau, out, false);
   for (int k = 0; k < nv; ++k) vfree[k] = v[k] + h * out[k];
   world_frames(a, lane);
   // 2. contacts: points inside a half-space, then touching spheres
