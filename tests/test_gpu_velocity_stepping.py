@@ -315,3 +315,37 @@ def test_interface_simulator_drops_a_cube(gp):
         q = sim.step(1.0 / 60.0, [])
     assert q.shape == (7,) and abs(q[6] - 0.25) < 1e-6
     assert abs(sim.contact_positions().reshape(-1, 3)[:, 2].min()) < 1e-6
+
+
+def test_thrown_cubes_at_full_size(gp, oracle):
+    """The benchmark's workload at its size (16,384 cubes thrown at the
+    ground, CCD velocity stepping, dt 1/120, one second): no fault, no corner
+    below the ground after any step (the collision loop's guarantee,
+    collision/ccd/mod.rs:284-337), every cube at rest on a face at the end,
+    and 64 sampled cubes checked step by step against the oracle."""
+    from gorilla_physics_amd import batch, workloads
+    print("case thrown cubes 16384", flush=True)
+    w = workloads.thrown_cubes(n=16384)
+    d = w["models"][0]
+    b = batch.Batch([batch.Model(d)], w["counts"], precision=0)
+    b.set_state(w["q"], w["v"])
+    corners = np.array([[sx, sy, sz] for sx in (-0.05, 0.05) for sy in (-0.05, 0.05) for sz in (-0.05, 0.05)])
+    sample = np.random.default_rng(5).choice(16384, 64, replace=False)
+    q, v = w["q"].copy(), w["v"].copy()
+    worst, lowest = 0.0, np.inf
+    for k in range(w["steps"]):
+        b.step(w["dt"], CCD, 1)
+        qn, vn = b.get_state()
+        Q = qn.reshape(-1, 7)
+        if k % 8 == 0:
+            for i in sample:
+                rq, rv, _ = oracle.vs_step(d, q[7 * i:7 * i + 7], v[6 * i:6 * i + 6], None, w["dt"], True)
+                worst = max(worst, _rel(qn[7 * i:7 * i + 7], rq), _rel(vn[6 * i:6 * i + 6], rv))
+        for i in range(0, 16384, 97):  # every 97th cube's lowest corner
+            R = gp.UnitQuaternion(*Q[i, :4]).to_matrix()
+            lowest = min(lowest, float((corners @ R.T)[:, 2].min() + Q[i, 6]))
+        q, v = qn, vn
+    print(f"worst per-step difference {worst:.3g}, lowest corner {lowest:.3g}", flush=True)
+    assert not b.faults().any()
+    assert worst < TOL and lowest > -1e-8
+    assert np.abs(q.reshape(-1, 7)[:, 6] - 0.05).max() < 1e-6 and np.abs(v).max() < 1e-6
