@@ -152,6 +152,7 @@ SIGNATURES = {
                               C.c_int64, C.POINTER(C.c_int64)]),
     "gp_batch_faults": (C.c_int, [_P, C.POINTER(C.c_uint8)]),
     "gp_batch_clear_faults": (C.c_int, [_P]),
+    "gp_batch_collision_rounds": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "gp_batch_last_timing": (C.c_int, [_P, _D, C.POINTER(C.c_int64)]),
     "gp_batch_set_state_dev": (C.c_int, [_P, C.c_void_p, C.c_void_p, _P]),
     "gp_batch_get_state_dev": (C.c_int, [_P, C.c_void_p, C.c_void_p, _P]),

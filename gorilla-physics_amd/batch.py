@@ -171,6 +171,13 @@ class Batch:
     def clear_faults(self):
         abi.check(self.lib.gp_batch_clear_faults(self.handle), self.lib)
 
+    def collision_rounds(self):
+        """Rounds of the collision loop in each instance's last
+        velocity-stepping step (1 = not cut; 0 = no such step yet)."""
+        r = np.zeros(self.n_instances, dtype=np.int32)
+        abi.check(self.lib.gp_batch_collision_rounds(self.handle, r.ctypes.data_as(C.POINTER(C.c_int32))), self.lib)
+        return r
+
     # device-array transfers: q, v, tau, vdot are device addresses (ints), e.g.
     # torch tensors' data_ptr(); stream is a hipStream_t address or None
     def set_state_dev(self, q_ptr, v_ptr, stream=None):

@@ -129,6 +129,7 @@ struct BucketArgs {
   const double* tau;   // [nv][Bp]
   double* vdot;        // [nv][Bp] acceleration of the last step
   int* fault;          // [Bp]
+  int* rounds;         // [Bp] collision-loop rounds of the last velocity-stepping step
   int* active;         // [Bp] or null: per-lane remaining steps (serving);
                        // the FAST kernels count it down by the steps they run
   double* scratch;     // plain kernel: [NPF][nb][Bp] + RK stages

@@ -312,7 +312,7 @@ void Bucket::release() {
   d_sphere_of = nullptr;
   d_shp = nullptr;
   scratch_doubles = 0;
-  d_parent = d_qoff = d_voff = d_cstart = d_kind = d_fault = d_active = nullptr;
+  d_parent = d_qoff = d_voff = d_cstart = d_kind = d_fault = d_active = d_rounds = nullptr;
   d_bp = d_cp = d_hp = d_q = d_v = d_tau = d_tau_tmp = d_vdot = d_scratch = nullptr;
   d_bpf = d_cpf = d_hpf = nullptr;
 }
@@ -347,7 +347,7 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
                o_v = take(8 * nv1 * Bp), o_tau = take(8 * nv1 * Bp), o_tau_tmp = take(8 * nv1 * Bp),
                o_vdot = take(8 * nv1 * Bp), o_fault = take(4 * Bp), o_sstart = take(4 * (b.nb + 1)),
                o_sp = take(8 * NSF * b.nsp * Bp), o_ss = take(8 * NSS * b.nsp * Bp),
-               o_sphere_of = take(4 * b.nb), o_shp = take(8 * NSH * b.ns * Bp);
+               o_sphere_of = take(4 * b.nb), o_shp = take(8 * NSH * b.ns * Bp), o_rounds = take(4 * Bp);
   std::vector<char> h(at, 0);
   auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(h.data() + off, src, bytes); };
   put(o_parent, b.parent.data(), 4 * b.nb);
@@ -412,7 +412,7 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
   b.d_q = (double*)(d + o_q); b.d_v = (double*)(d + o_v); b.d_tau = (double*)(d + o_tau);
   b.d_tau_tmp = (double*)(d + o_tau_tmp); b.d_vdot = (double*)(d + o_vdot); b.d_fault = (int*)(d + o_fault);
   b.d_sstart = (int*)(d + o_sstart); b.d_sp = (double*)(d + o_sp); b.d_ss = (double*)(d + o_ss);
-  b.d_sphere_of = (int*)(d + o_sphere_of); b.d_shp = (double*)(d + o_shp);
+  b.d_sphere_of = (int*)(d + o_sphere_of); b.d_shp = (double*)(d + o_shp); b.d_rounds = (int*)(d + o_rounds);
   b.arena_bytes = at;
   return GP_OK;
 }
@@ -448,6 +448,7 @@ BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
   a.parent = b.d_parent; a.qoff = b.d_qoff; a.voff = b.d_voff; a.cstart = b.d_cstart;
   a.kind = b.d_kind; a.bp = b.d_bp; a.bpf = b.d_bpf; a.cp = b.d_cp; a.cpf = b.d_cpf; a.hp = b.d_hp; a.hpf = b.d_hpf;
   a.q = b.d_q; a.v = b.d_v; a.tau = tau ? tau : b.d_tau; a.vdot = b.d_vdot; a.fault = b.d_fault;
+  a.rounds = b.d_rounds;
   a.active = b.d_active; a.scratch = b.d_scratch;
   a.nsp = b.nsp; a.sstart = b.d_sstart; a.sp = b.d_sp; a.ss = b.d_ss;
   a.ns = b.ns; a.sphere_of = b.d_sphere_of; a.shp = b.d_shp; a.no_springs = 0;
@@ -1028,6 +1029,17 @@ int gp_batch_faults(gp_batch* bt, uint8_t* faults) {
     std::vector<int> f(b.Bp);
     HIP_TRY(hipMemcpy(f.data(), b.d_fault, sizeof(int) * b.Bp, hipMemcpyDeviceToHost));
     for (int l = 0; l < b.B; ++l) faults[b.inst[l]] = f[l] ? 1 : 0;
+  }
+  return GP_OK;
+}
+
+int gp_batch_collision_rounds(gp_batch* bt, int32_t* rounds) {
+  if (!bt || !rounds) return fail(GP_ERR_INVALID, "null argument");
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  for (Bucket& b : bt->buckets) {
+    std::vector<int> r(b.Bp);
+    HIP_TRY(hipMemcpy(r.data(), b.d_rounds, sizeof(int) * b.Bp, hipMemcpyDeviceToHost));
+    for (int l = 0; l < b.B; ++l) rounds[b.inst[l]] = r[l];
   }
   return GP_OK;
 }

@@ -53,6 +53,7 @@ struct Bucket {
   float *d_bpf = nullptr, *d_cpf = nullptr, *d_hpf = nullptr;
   double *d_q = nullptr, *d_v = nullptr, *d_tau = nullptr, *d_tau_tmp = nullptr, *d_vdot = nullptr;
   int* d_fault = nullptr;
+  int* d_rounds = nullptr;
   int* d_active = nullptr;
   double* d_scratch = nullptr;
   size_t scratch_doubles = 0;
@@ -79,6 +80,8 @@ struct Bucket {
       d_parent = o.d_parent; d_qoff = o.d_qoff; d_voff = o.d_voff; d_cstart = o.d_cstart; d_kind = o.d_kind;
       d_bp = o.d_bp; d_cp = o.d_cp; d_hp = o.d_hp; d_bpf = o.d_bpf; d_cpf = o.d_cpf; d_hpf = o.d_hpf;
       d_q = o.d_q; d_v = o.d_v; d_tau = o.d_tau; d_tau_tmp = o.d_tau_tmp; d_vdot = o.d_vdot; d_fault = o.d_fault;
+      d_rounds = o.d_rounds;
+      o.d_rounds = nullptr;
       d_active = o.d_active; d_scratch = o.d_scratch; d_arena = o.d_arena; arena_bytes = o.arena_bytes;
       d_flat_off = o.d_flat_off;
       o.d_arena = nullptr;

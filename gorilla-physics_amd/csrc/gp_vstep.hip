@@ -588,9 +588,10 @@ __global__ void __launch_bounds__(64) vs_step_kernel(BucketArgs a, double dt, in
   for (int k = 0; k < nv; ++k) vc[k] = v[k];
   bool ok = true, done = false;
   double t_remain = dt;
-  int forced_body = -1, forced_h = -1;
+  int forced_body = -1, forced_h = -1, rounds = 0;
   for (int round = 0; round < kCcdMaxRounds && !done; ++round) {
     int nc = 0, hit_body = -1, hit_h = -1;
+    rounds = round + 1;
     ok = vs_dynamics(V, qc, vc, tau, t_remain, &nc, forced_body, forced_h) && ok;
     const double earliest = ccd ? earliest_collision(V, nc, &hit_body, &hit_h) : 1e300;
     if (earliest < t_remain) {
@@ -613,6 +614,7 @@ __global__ void __launch_bounds__(64) vs_step_kernel(BucketArgs a, double dt, in
   for (int k = 0; k < nq; ++k) q[k] = qc[k];
   ok = ok && finite_cols(q, nq) && finite_cols(v, nv);
   if (!ok) a.fault[lane] = 1;
+  a.rounds[lane] = rounds;
 }
 
 size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp) {

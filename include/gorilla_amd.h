@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GP_ABI_VERSION 3
+#define GP_ABI_VERSION 4
 
 /* ---- status codes (replace the reference's panic!/expect sites) ---------- */
 enum {
@@ -286,6 +286,14 @@ int gp_batch_energy(gp_batch* batch, double* kinetic, double* gravitational);
  * contact.rs:161-163). */
 int gp_batch_faults(gp_batch* batch, uint8_t* faults);
 int gp_batch_clear_faults(gp_batch* batch);
+
+/* Rounds of the collision loop in each instance's last velocity-stepping step
+ * (the loop of ccd_velocity_stepping, integrators.rs:56-174: 1 = the step
+ * was not cut; always 1 without continuous collision detection); 0 for an
+ * instance that has taken no such step since the batch was built. A
+ * diagnostic with no counterpart in the reference, whose loop count is
+ * internal. */
+int gp_batch_collision_rounds(gp_batch* batch, int32_t* rounds);
 
 /* Time the batch's last gp_step on its own stream (HIP events around every
  * launch): total milliseconds and number of kernel launches. */
