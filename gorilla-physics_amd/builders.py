@@ -1,9 +1,11 @@
-"""Model builders mirroring the reference's helpers.rs (same names and
-arguments), used by the tests and the workload generators."""
+"""Model builders mirroring the reference's helpers.rs and builders/ (same
+names and arguments), used by the tests and the workload generators. Masses,
+sizes, frames and joint axes are the reference's: parity is defined on these
+models."""
 import numpy as np
 
-from .mechanism import (WORLD_FRAME, ContactPoint, FloatingJoint, Isometry3, MechanismState, PrismaticJoint,
-                        RevoluteJoint, RigidBody, SpatialInertia, SpringContact, Transform3D)
+from .mechanism import (WORLD_FRAME, ContactPoint, FloatingJoint, Isometry3, JointSpring, MechanismState,
+                        PrismaticJoint, RevoluteJoint, RigidBody, SpatialInertia, SpringContact, Transform3D)
 
 
 def build_pendulum(mass, moment, cross_part, rod_to_world, axis):  # helpers.rs:24
@@ -99,3 +101,142 @@ def build_quadruped():  # helpers.rs:423 (moments as written there)
     for name in ("fr", "fl", "br", "bl"):
         state.add_contact_point(ContactPoint.new_with_k(f"{name}_knee", [0.0, 0.0, -l_leg / 2.0], 10e3))
     return state
+
+
+def build_rimless_wheel(m_body, r_body, l, n_foot):  # helpers.rs:168
+    """A solid-sphere body (moment 2/5 m r^2, no collider) on a floating
+    joint with n_foot spoke tips at distance l, spaced 2 pi / n_foot about y,
+    the first straight down."""
+    mx = 2.0 / 5.0 * m_body * r_body * r_body
+    body = RigidBody(SpatialInertia("body", np.diag([mx, mx, mx]), np.zeros(3), m_body))
+    state = MechanismState([FloatingJoint(Transform3D.identity("body", WORLD_FRAME))], [body])
+    for i in range(n_foot):
+        a = i * 2.0 * np.pi / n_foot
+        state.add_contact_point(ContactPoint.new("body", [-l * np.sin(a), 0.0, -l * np.cos(a)]))
+    return state
+
+
+def build_2d_hopper(m_body, w_body, h_body, m_hip, r_hip, body_hip_length, m_piston, r_piston,
+                    hip_piston_length, m_leg, l_leg, piston_leg_length):  # helpers.rs:203
+    """Floating body, revolute hip about y, prismatic piston and prismatic leg
+    along -z, one contact point at the leg's frame origin. The body's z
+    moment is (w^2 + w^2) m / 12 as the reference writes it."""
+    moment_body = np.diag([(w_body * w_body + h_body * h_body) * m_body / 12.0,
+                           (w_body * w_body + h_body * h_body) * m_body / 12.0,
+                           (w_body * w_body + w_body * w_body) * m_body / 12.0])
+
+    def ball(frame, m, r):
+        mx = 2.0 / 5.0 * m * r * r
+        return RigidBody(SpatialInertia(frame, np.diag([mx, mx, mx]), np.zeros(3), m))
+
+    ml = 1.0 / 3.0 * m_leg * l_leg * l_leg
+    bodies = [RigidBody(SpatialInertia("body", moment_body, np.zeros(3), m_body)), ball("hip", m_hip, r_hip),
+              ball("piston", m_piston, r_piston),
+              RigidBody(SpatialInertia("leg", np.diag([ml, ml, 0.0]), np.array([0.0, 0.0, m_leg * l_leg / 2.0]),
+                                       m_leg))]
+    joints = [FloatingJoint(Transform3D.identity("body", WORLD_FRAME)),
+              RevoluteJoint(Transform3D.move_z("hip", "body", -body_hip_length), [0.0, 1.0, 0.0]),
+              PrismaticJoint(Transform3D.move_z("piston", "hip", -hip_piston_length), [0.0, 0.0, -1.0]),
+              PrismaticJoint(Transform3D.move_z("leg", "piston", -piston_leg_length), [0.0, 0.0, -1.0])]
+    state = MechanismState(joints, bodies)
+    state.add_contact_point(ContactPoint.new("leg", [0.0, 0.0, 0.0]))
+    return state
+
+
+def build_hopper(m_foot, r_foot, m_hip, r_hip, m_body, r_body, l_foot_to_hip):  # helpers.rs:345
+    """Three spheres (with their colliders): a floating foot, a hip on a
+    sprung prismatic leg along z (k 1e3, rest length 0), a body on a revolute
+    joint about y; the foot's contact point has k = 75e3."""
+    joints = [FloatingJoint(Transform3D.identity("foot", WORLD_FRAME)),
+              PrismaticJoint.new_with_spring(Transform3D.move_z("hip", "foot", l_foot_to_hip), [0.0, 0.0, 1.0],
+                                             JointSpring(1e3, 0.0)),
+              RevoluteJoint(Transform3D.identity("body", "hip"), [0.0, 1.0, 0.0])]
+    bodies = [RigidBody.new_sphere(m_foot, r_foot, "foot"), RigidBody.new_sphere(m_hip, r_hip, "hip"),
+              RigidBody.new_sphere(m_body, r_body, "body")]
+    state = MechanismState(joints, bodies)
+    state.add_contact_point(ContactPoint.new_with_k("foot", [0.0, 0.0, 0.0], 75e3))
+    return state
+
+
+_L1, _L2, _M_LINK = 0.05, 0.2, 0.1  # link thickness, link length, link mass of the biped family
+
+
+def _biped_leg(side, sign, with_ankle, limb_contacts, foot_axis):
+    """One leg of build_biped / build_leg (biped_builder.rs:12-188,
+    leg_builder.rs:8-109): pelvis (revolute z) at x = sign (L1 + L1) / 2
+    from the base, hip (revolute -y) below the pelvis, thigh (revolute x) at
+    the hip's outer end, calf (revolute x), an optional ankle (revolute x),
+    foot with its eight corners as contact points."""
+    L1, L2, m = _L1, _L2, _M_LINK
+    down = [0.0, 0.0, -L2 / 2.0]
+    names = [f"{n}_{side}" for n in ("pelvis", "hip", "thigh", "calf", "ankle", "foot")]
+    pelvis, hip, thigh, calf, ankle, foot = names
+    bodies = [RigidBody.new_cuboid_at(down, m, L1, L1, L2, pelvis),
+              RigidBody.new_cuboid_at([sign * L2 / 2.0, 0.0, 0.0], m, L2, L1, L1, hip),
+              RigidBody.new_cuboid_at(down, m, L1, L1, L2, thigh),
+              RigidBody.new_cuboid_at(down, m, L1, L1, L2, calf)]
+    joints = [RevoluteJoint(Transform3D.move_x(pelvis, "base", sign * (L1 + L1) / 2.0), [0.0, 0.0, 1.0]),
+              RevoluteJoint(Transform3D.move_xyz(hip, pelvis, 0.0, 0.0, -L2), [0.0, -1.0, 0.0]),
+              RevoluteJoint(Transform3D.move_x(thigh, hip, sign * L2), [1.0, 0.0, 0.0]),
+              RevoluteJoint(Transform3D.move_z(calf, thigh, -L2), [1.0, 0.0, 0.0])]
+    if limb_contacts:
+        for b in bodies[2:4]:
+            b.add_cuboid_contacts_with(down, L1, L1, L2)
+    above = calf
+    if with_ankle:
+        bodies.append(RigidBody.new_cuboid(m, L1, L2, L1, ankle))
+        joints.append(RevoluteJoint(Transform3D.move_z(ankle, calf, -L2), [1.0, 0.0, 0.0]))
+        above = ankle
+    f = RigidBody.new_cuboid(m, 0.2, L2, L1, foot)
+    f.add_cuboid_contacts(0.2, L2, L1)
+    bodies.append(f)
+    offset = -(L1 + L1) / 2.0 if with_ankle else -L2
+    joints.append(RevoluteJoint(Transform3D.move_z(foot, above, offset), foot_axis))
+    return joints, bodies
+
+
+def _biped_base():
+    return (FloatingJoint(Transform3D.identity("base", WORLD_FRAME)),
+            RigidBody.new_cuboid(_M_LINK, _L1, _L1, _L2, "base"))
+
+
+def build_biped():  # builders/biped_builder.rs:12
+    """Floating base and two six-joint legs (pelvis, hip, thigh, calf,
+    ankle, foot); 13 bodies, 16 contact points (the feet's corners)."""
+    jb, bb = _biped_base()
+    jl, bl = _biped_leg("left", 1.0, True, False, [0.0, -1.0, 0.0])
+    jr, br = _biped_leg("right", -1.0, True, False, [0.0, -1.0, 0.0])
+    return MechanismState([jb] + jl + jr, [bb] + bl + br)
+
+
+def build_leg():  # builders/leg_builder.rs:8
+    """Floating base and one five-joint leg (pelvis, hip, thigh, calf, foot
+    about x), with contact points on the thigh, calf and foot corners."""
+    jb, bb = _biped_base()
+    jl, bl = _biped_leg("left", 1.0, False, True, [1.0, 0.0, 0.0])
+    return MechanismState([jb] + jl, [bb] + bl)
+
+
+def build_leg_from_foot():  # builders/leg_builder.rs:111
+    """The leg rooted at a floating foot: calf, thigh (revolute x), hip
+    (revolute x), pelvis (revolute -y), base (revolute z), each link's centre
+    of mass on the side away from the foot; contact points on the foot,
+    calf and thigh corners."""
+    L1, L2, m = _L1, _L2, _M_LINK
+    up = [0.0, 0.0, L2 / 2.0]
+    foot = RigidBody.new_cuboid(m, 0.2, L2, L1, "foot_left")
+    foot.add_cuboid_contacts(0.2, L2, L1)
+    calf = RigidBody.new_cuboid_at(up, m, L1, L1, L2, "calf_left")
+    calf.add_cuboid_contacts_with(up, L1, L1, L2)
+    thigh = RigidBody.new_cuboid_at(up, m, L1, L1, L2, "thigh_left")
+    thigh.add_cuboid_contacts_with(up, L1, L1, L2)
+    hip = RigidBody.new_cuboid_at([-L2 / 2.0, 0.0, 0.0], m, L2, L1, L1, "hip_left")
+    pelvis = RigidBody.new_cuboid_at(up, m, L1, L1, L2, "pelvis_left")
+    base = RigidBody.new_cuboid_at([-(L1 + L1) / 2.0, 0.0, 0.0], m, L1, L1, L2, "base")
+    joints = [FloatingJoint(Transform3D.identity("foot_left", WORLD_FRAME)),
+              RevoluteJoint(Transform3D.identity("calf_left", "foot_left"), [1.0, 0.0, 0.0]),
+              RevoluteJoint(Transform3D.move_z("thigh_left", "calf_left", L2), [1.0, 0.0, 0.0]),
+              RevoluteJoint(Transform3D.move_z("hip_left", "thigh_left", L2), [1.0, 0.0, 0.0]),
+              RevoluteJoint(Transform3D.move_x("pelvis_left", "hip_left", -L2), [0.0, -1.0, 0.0]),
+              RevoluteJoint(Transform3D.move_z("base", "pelvis_left", L2), [0.0, 0.0, 1.0])]
+    return MechanismState(joints, [foot, calf, thigh, hip, pelvis, base])

@@ -9,7 +9,7 @@ no torques (zero torques).
 """
 import numpy as np
 
-from .mechanism import HalfSpace, Integrator, flatten_q, step
+from .mechanism import HalfSpace, Integrator, Pose, flatten_q, step
 
 
 class ControlInput:  # control/mod.rs:22-30
@@ -20,6 +20,44 @@ class ControlInput:  # control/mod.rs:22-30
 class NullController:  # control/mod.rs:40-48
     def control(self, state, input=None):
         return []
+
+
+def _standing(state, q_joints, translation):
+    """Set the floating root to the identity rotation at `translation` and
+    the remaining joints to q_joints."""
+    state.set_joint_q(1, Pose(None, translation))
+    for k, a in enumerate(q_joints):
+        state.set_joint_q(k + 2, float(a))
+    return state
+
+
+def create_biped():  # interface/biped.rs:16-56
+    """build_biped with both legs bent (thigh -pi/4, calf pi/2, ankle -pi/4),
+    lowered so that the last body's (the right foot's) frame origin is at
+    z = 0."""
+    from .builders import build_biped
+    leg = [0.0, 0.0, -np.pi / 4.0, np.pi / 2.0, -np.pi / 4.0, 0.0]
+    state = _standing(build_biped(), leg + leg, [0.0, 0.0, 0.0])
+    foot = state.poses()[-1].translation
+    return _standing(state, leg + leg, [0.0, 0.0, -float(foot[2])])
+
+
+def create_leg():  # interface/biped.rs:59-95
+    """build_leg bent the same way (thigh -pi/4, calf pi/2, foot -pi/4),
+    moved so that the foot's frame origin is at the world origin."""
+    from .builders import build_leg
+    leg = [0.0, 0.0, -np.pi / 4.0, np.pi / 2.0, -np.pi / 4.0]
+    state = _standing(build_leg(), leg, [0.0, 0.0, 0.0])
+    foot = np.asarray(state.poses()[-1].translation, dtype=float)
+    return _standing(state, leg, list(-foot))
+
+
+def create_leg_from_foot():  # interface/biped.rs:98-124
+    """build_leg_from_foot with the foot at the origin and calf pi/4 - 0.4,
+    thigh -pi/2, hip pi/4, pelvis 0.1, base pi/2."""
+    from .builders import build_leg_from_foot
+    return _standing(build_leg_from_foot(), [np.pi / 4.0 - 0.4, -np.pi / 2.0, np.pi / 4.0, 0.1, np.pi / 2.0],
+                     [0.0, 0.0, 0.0])
 
 
 class InterfaceSimulator:

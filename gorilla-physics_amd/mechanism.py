@@ -315,6 +315,25 @@ class RigidBody:
     def add_sphere_collider(self, sphere):  # rigid_body.rs:201
         self.collider = Collider.new_sphere(sphere)
 
+    def add_cuboid_contacts(self, w, d, h):  # rigid_body.rs:216
+        """The eight corners of a w x d x h box centred on the frame origin:
+        the top face (z = +h/2) first, each face in the order (-x, +y),
+        (+x, +y), (-x, -y), (+x, -y)."""
+        for z in (h / 2.0, -h / 2.0):
+            for y in (d / 2.0, -d / 2.0):
+                for x in (-w / 2.0, w / 2.0):
+                    self.add_contact_point(ContactPoint.new(self.inertia.frame, [x, y, z]))
+
+    def add_cuboid_contacts_with(self, com, w, d, h):  # rigid_body.rs:251
+        """The eight corners of a w x d x h box centred on com, x slowest and
+        z fastest."""
+        c = np.asarray(com, dtype=float)
+        for i in (-1.0, 1.0):
+            for j in (-1.0, 1.0):
+                for k in (-1.0, 1.0):
+                    self.add_contact_point(ContactPoint.new(self.inertia.frame,
+                                                            c + [i * w / 2.0, j * d / 2.0, k * h / 2.0]))
+
 
 # --------------------------------------------------------------------- joints
 class JointSpring:

@@ -1,0 +1,99 @@
+"""The reference's model builders (helpers.rs, builders/) through the HIP
+path: every model in one batch, each instance started in contact with a
+ground plane with random joint velocities, and every step of every instance
+compared with one oracle step from the same state (FP64: <= 1e-10, the
+bound of the other FP64 parity tests); the biped standing as the browser
+client creates it (interface/biped.rs:16-56), stepped by velocity stepping
+with continuous collision detection, against the oracle at its 1e-5."""
+import numpy as np
+import pytest
+
+from gpu_helpers import step_errors
+from test_gpu_velocity_stepping import CCD, VS, Stepper
+from test_oracle_golden import builder_models
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-10
+
+
+def _on_ground(gp, name, rng):
+    """The builder's model (the browser client's standing pose where it has
+    one) on a plane tilted 0.1 rad about y whose offset puts the lowest
+    contact point 1 mm inside it, with random velocities."""
+    from gorilla_physics_amd import interface
+    create = {"leg": interface.create_leg, "leg_from_foot": interface.create_leg_from_foot,
+              "biped": interface.create_biped}
+    st = create[name]() if name in create else builder_models(gp)[name][0]
+    n = np.array([np.sin(0.1), 0.0, np.cos(0.1)])
+    lowest = (interface.InterfaceSimulator(st).contact_positions().reshape(-1, 3) @ n).min()
+    st.add_halfspace(gp.HalfSpace.new_with_params(n, lowest + 1e-3, 0.9, 0.5))
+    return st.model_desc(), st.flat_q(), st.flat_v() + rng.normal(size=st.flat_v().size) * 0.3
+
+
+@pytest.mark.parametrize("integrator", [0, 2])
+def test_builder_models_step_parity(gp, oracle, integrator):
+    from gorilla_physics_amd import batch
+    rng = np.random.default_rng(11)
+    descs, q, v = [], [], []
+    for name in builder_models(gp):
+        for _ in range(3):
+            d, qi, vi = _on_ground(gp, name, rng)
+            descs.append(d)
+            q.append(qi)
+            v.append(vi)
+    print(f"case builder models, integrator {integrator}, {len(descs)} instances", flush=True)
+    b = batch.Batch([batch.Model(d) for d in descs], [1] * len(descs), precision=0)
+    q, v = np.concatenate(q), np.concatenate(v)
+    tau = np.zeros(v.size)
+    b.set_state(q, v)
+    worst = 0.0
+    for k in range(200):
+        b.step(1e-3, integrator, 1)
+        q1, v1 = b.get_state()
+        a1 = b.get_vdot()
+        if k % 5 == 0:
+            eq, ev, ea = step_errors(oracle, descs, q, v, tau, q1, v1, a1, 1e-3, integrator)
+            worst = max(worst, eq.max(), ev.max(), ea.max())
+        q, v = q1, v1
+    print(f"worst per-step difference {worst:.3g}", flush=True)
+    assert not b.faults().any()
+    assert worst <= F64_TOL
+
+
+def test_interface_biped_velocity_stepping(gp, oracle):
+    """The standing biped of createBiped on a plane through its feet's
+    bottom faces, pushed sideways, stepped by velocity stepping at 1/60 s
+    against the oracle (and the same number of collision-loop rounds).
+
+    With continuous collision detection the reference's loop diverges on
+    this model (feet of 0.1 kg links, each cut adding gravity for the rest of
+    the step again, integrators.rs:56-60): the oracle's velocities pass 1e269
+    within the second step at 1/60 s, and at 1/240 to 1/1000 s it runs away or
+    exceeds 64 rounds within 300 steps. The GPU must flag that instance at
+    the step where the oracle's state turns non-finite, and it is stepped
+    without CCD here."""
+    from gorilla_physics_amd import interface
+    def standing():
+        st = interface.create_biped()
+        st.add_halfspace(gp.HalfSpace.new([0, 0, 1.0], -0.025))
+        v = st.flat_v()
+        v[3] = 0.3
+        return st, v
+    st, v = standing()
+    print("case interface biped, velocity stepping", flush=True)
+    s = Stepper(oracle, [st.model_desc()], st.flat_q(), v, integrator=VS)
+    for _ in range(60):
+        s.step(1.0 / 60.0)
+    print(f"worst per-step difference {s.worst:.3g}", flush=True)
+    assert not s.b.faults().any()
+    st, v = standing()
+    print("case interface biped, CCD velocity stepping: the fault at step 1", flush=True)
+    s = Stepper(oracle, [st.model_desc()], st.flat_q(), v, integrator=CCD)
+    s.step(1.0 / 60.0)
+    assert not s.b.faults().any()
+    q1, v1 = s.q, s.v
+    rq, rv, _ = oracle.vs_step(st.model_desc(), q1, v1, None, 1.0 / 60.0, True)
+    assert not (np.isfinite(rq).all() and np.isfinite(rv).all())
+    s.step(1.0 / 60.0, check=False)
+    assert s.b.faults().all()

@@ -109,3 +109,23 @@ def test_interface_poses_and_contact_positions(gp):
                        sorted((x, y, z) for x in (-1.0, 1.0) for y in (-1.0, 1.0) for z in (-1.0, 1.0)))
     sim.addHalfSpace([0, 0, 2.0], 0.5)
     assert np.allclose(st.halfspaces[0].normal, [0, 0, 1]) and np.allclose(st.halfspaces[0].point, [0, 0, 0.5])
+
+
+def test_interface_biped_and_leg_poses(gp):
+    """createBiped lowers the biped until the right foot's frame origin is
+    at z = 0 and createLeg moves the foot's origin to the world origin
+    (interface/biped.rs:16-95); with the legs bent (thigh -pi/4, calf pi/2)
+    the base stands pelvis (0.2) + ankle-to-foot (0.05) + thigh and calf at
+    45 degrees (2 x 0.2 cos(pi/4)) above the feet."""
+    from gorilla_physics_amd import interface
+    st = interface.create_biped()
+    p = st.poses()
+    names = [b.inertia.frame for b in st.bodies]
+    assert abs(p[-1].translation[2]) < 1e-15
+    base = p[names.index("base")].translation
+    assert abs(base[2] - (0.2 + 0.05 + 2 * 0.2 * math.cos(math.pi / 4))) < 1e-12
+    assert np.allclose(p[names.index("foot_left")].translation[2], p[names.index("foot_right")].translation[2])
+    st = interface.create_leg()
+    assert np.abs(st.poses()[-1].translation).max() < 1e-15
+    st = interface.create_leg_from_foot()
+    assert np.abs(st.poses()[0].translation).max() == 0.0 and st.q[1] == math.pi / 4 - 0.4
