@@ -97,3 +97,70 @@ def test_interface_biped_velocity_stepping(gp, oracle):
     assert not (np.isfinite(rq).all() and np.isfinite(rv).all())
     s.step(1.0 / 60.0, check=False)
     assert s.b.faults().all()
+
+
+@pytest.mark.parametrize("integrator", [0, 2, 3])
+def test_fixed_joint_arm_step_parity(gp, oracle, integrator):
+    """An arm with a fixed joint between its two revolutes (the oracle's
+    fixed-joint handling is pinned by the merged-body identity), 32 random
+    instances, some touching the plane: every step against the oracle
+    (velocity stepping at its 1e-5, the others at 1e-10)."""
+    from gorilla_physics_amd import batch
+    from test_oracle_golden import welded_arm
+    print(f"case welded arm, integrator {integrator}", flush=True)
+    d = welded_arm(gp).model_desc()
+    rng = np.random.default_rng(8)
+    n = 32
+    q, v = rng.uniform(-2, 2, 2 * n), rng.normal(size=2 * n)
+    if integrator == 3:
+        s = Stepper(oracle, [d] * n, q, v, integrator=VS)
+        for _ in range(50):
+            s.step(1e-2)
+        print(f"worst per-step difference {s.worst:.3g}", flush=True)
+        assert not s.b.faults().any()
+        return
+    b = batch.Batch([batch.Model(d)], [n], precision=0)
+    b.set_state(q, v)
+    tau = np.zeros(2 * n)
+    worst = 0.0
+    for k in range(100):
+        b.step(1e-3, integrator, 1)
+        q1, v1 = b.get_state()
+        a1 = b.get_vdot()
+        eq, ev, ea = step_errors(oracle, [d] * n, q, v, tau, q1, v1, a1, 1e-3, integrator)
+        worst = max(worst, eq.max(), ev.max(), ea.max())
+        q, v = q1, v1
+    print(f"worst per-step difference {worst:.3g}", flush=True)
+    assert not b.faults().any() and worst <= F64_TOL
+
+
+def test_fixed_joint_no_move_through_the_mirror(gp):  # joint/fixed.rs:45,65
+    """A cube welded to the world has no coordinates; a second of velocity
+    stepping (and of semi-implicit Euler) through the HIP path leaves it
+    where the joint puts it and returns empty states."""
+    from test_oracle_golden import fixed_cube
+    print("case fixed cube", flush=True)
+    for integ in (gp.Integrator.VelocityStepping, gp.Integrator.SemiImplicitEuler):
+        st, iso = fixed_cube(gp)
+        for _ in range(100):
+            q, v = gp.step(st, 1e-2, [], integ)
+        assert st.flat_q().size == 0 and st.flat_v().size == 0
+        pose = st.poses()[0]
+        assert np.allclose(pose.translation, iso.translation, atol=1e-12)
+        assert np.allclose(pose.rotation.coords(), iso.rotation.coords(), atol=1e-12)
+
+
+def test_ball_fall(gp, oracle):  # joint/floating.rs:133
+    from gorilla_physics_amd import batch
+    from test_oracle_golden import ball_fall, check_ball_fall
+    print("case reference ball_fall", flush=True)
+
+    def gpu_simulate(d, final_time, dt, q0, v0, tau, integrator):
+        b = batch.Batch([batch.Model(d)], [1], precision=0)
+        b.set_state(np.asarray(q0, float), np.asarray(v0, float))
+        return b.simulate(final_time, dt, None, integrator)
+
+    q0, v0, qs = ball_fall(gp, gpu_simulate)
+    check_ball_fall(qs[-1])
+    _, _, rq = ball_fall(gp, oracle.simulate)
+    assert qs.shape == rq.shape and np.abs(qs - rq).max() < 1e-12

@@ -493,6 +493,131 @@ def test_biped_corner_order_and_leg_frames(gp):
         assert np.allclose(foot, [sx + 4 * sx, 0.0, -0.2 - 0.2 - 0.2 - 0.05]), (side, foot)
 
 
+def fixed_cube(gp):  # joint/fixed.rs:65-76: a cube welded to the world at (1, 0, 0), rotated 1 rad about x
+    iso = gp.Isometry3.translation_xyz(1.0, 0, 0) * gp.Isometry3.rotation_scaled_axis([1.0, 0, 0])
+    st = gp.MechanismState([gp.FixedJoint(gp.Transform3D.new("body", gp.WORLD_FRAME, iso))],
+                           [gp.RigidBody.new_cube(1.0, 1.0, "body")])
+    return st, iso
+
+
+def test_fixed_joint_no_velocity_and_no_move(gp, oracle):  # joint/fixed.rs:45,65
+    """A mechanism whose only joint is fixed has no coordinates: velocity
+    stepping returns an empty velocity, and its body stays at the joint's
+    transform over a second of steps."""
+    st, iso = fixed_cube(gp)
+    d = st.model_desc()
+    assert (d.nq, d.nv) == (0, 0)
+    q, v, _ = oracle.vs_step(d, np.zeros(0), np.zeros(0), None, 1e-1, False)
+    assert q.size == 0 and v.size == 0
+    qs, vs = oracle.simulate(d, 1.0, 1e-2, np.zeros(0), np.zeros(0), None, 3)
+    assert qs.shape[1] == 0 and len(qs) > 100
+    pose = st.poses()[0]
+    assert np.allclose(pose.translation, iso.translation, atol=1e-12)
+    assert np.allclose(pose.rotation.coords(), iso.rotation.coords(), atol=1e-12)
+
+
+def _inertia_in_parent(gp, inertia, iso, frame):
+    """A body's spatial inertia expressed in its parent's frame through the
+    fixed transform iso (child -> parent)."""
+    R = iso.rotation.to_matrix()
+    m = inertia.mass
+    c = np.asarray(inertia.cross_part) / m
+    Jc = np.asarray(inertia.moment) - m * (c.dot(c) * np.eye(3) - np.outer(c, c))
+    cp = R @ c + iso.translation
+    J = R @ Jc @ R.T + m * (cp.dot(cp) * np.eye(3) - np.outer(cp, cp))
+    return gp.SpatialInertia(frame, J, m * cp, m)
+
+
+def welded_arm(gp, merged=False):
+    """world -rev z- link1 -fixed (offset, 0.4 rad about x)- link2 -rev y-
+    link3, a contact point at link3's tip; merged: link1 and link2 as one
+    body carrying both inertias, link3 hung from it through the composed
+    transform."""
+    I1 = gp.SpatialInertia("l1", np.diag([0.02, 0.03, 0.04]), [0.05, 0.0, 0.1], 1.0)
+    I2 = gp.SpatialInertia("l2", np.diag([0.05, 0.01, 0.06]), [0.0, 0.12, -0.03], 0.7)
+    I3 = gp.SpatialInertia("l3", np.diag([0.04, 0.04, 0.01]), [0.0, 0.0, -0.2], 0.5)
+    weld = gp.Isometry3(gp.UnitQuaternion.from_axis_angle([1.0, 0, 0], 0.4), [0.3, 0.1, -0.2])
+    hang = gp.Isometry3.translation_xyz(0.0, 0.25, -0.1)
+    j1 = gp.RevoluteJoint(gp.Transform3D.new("l1", gp.WORLD_FRAME, gp.Isometry3.translation_xyz(0, 0, 1.0)), [0, 0, 1.0])
+    if merged:
+        I2p = _inertia_in_parent(gp, I2, weld, "l1")
+        both = gp.SpatialInertia("l1", np.asarray(I1.moment) + np.asarray(I2p.moment),
+                                 np.asarray(I1.cross_part) + np.asarray(I2p.cross_part), I1.mass + I2p.mass)
+        st = gp.MechanismState([j1, gp.RevoluteJoint(gp.Transform3D.new("l3", "l1", weld * hang), [0, 1.0, 0])],
+                               [gp.RigidBody(both), gp.RigidBody(I3)])
+    else:
+        st = gp.MechanismState([j1, gp.FixedJoint(gp.Transform3D.new("l2", "l1", weld)),
+                                gp.RevoluteJoint(gp.Transform3D.new("l3", "l2", hang), [0, 1.0, 0])],
+                               [gp.RigidBody(I1), gp.RigidBody(I2), gp.RigidBody(I3)])
+    st.add_contact_point(gp.ContactPoint.new("l3", [0.0, 0.0, -0.4]))
+    st.add_halfspace(gp.HalfSpace.new([0, 0, 1.0], 0.55))
+    return st
+
+
+@pytest.mark.parametrize("method", METHODS)
+def test_fixed_joint_equals_merged_body(gp, oracle, method):
+    """A fixed joint welds two bodies: the arm's accelerations equal those of
+    the arm with the two bodies merged into one (their inertias summed in one
+    frame), with and without the contact point touching the plane."""
+    d, dm = welded_arm(gp).model_desc(), welded_arm(gp, merged=True).model_desc()
+    assert (d.nq, d.nv, dm.nq) == (2, 2, 2)
+    rng = np.random.default_rng(4)
+    touching = 0
+    for _ in range(40):
+        q, v, tau = rng.uniform(-2, 2, 2), rng.normal(size=2), rng.normal(size=2)
+        a = oracle.dynamics(d, q, v, tau, method)
+        b = oracle.dynamics(dm, q, v, tau, method)
+        assert np.abs(a - b).max() < 1e-10 * max(1.0, np.abs(b).max()), (q, v)
+        touching += int(oracle.contact_distances(d, q).min() < 0)
+    assert 0 < touching < 40
+
+
+def test_ball_fall_rk4(gp, oracle):  # joint/floating.rs:133
+    """A ball turned 90 degrees about x, thrown up at 1 m/s (body y) while
+    spinning at 1 rad/s about x: the height follows the parabola, the
+    rotation stays about x and reaches pi/2 + 5 rad."""
+    q0, v0, qs = ball_fall(gp, oracle.simulate)
+    check_ball_fall(qs[-1])
+
+
+def ball_fall(gp, simulate):
+    m, r = 5.0, 1.0
+    mx = 2.0 / 5.0 * m * r * r
+    st = gp.MechanismState([gp.FloatingJoint(gp.Transform3D.identity("ball", gp.WORLD_FRAME))],
+                           [gp.RigidBody(gp.SpatialInertia("ball", np.diag([mx, mx, mx]), np.zeros(3), m))])
+    q0 = [*gp.UnitQuaternion.from_axis_angle([1.0, 0, 0], math.pi / 2).coords(), 0.0, 0.0, 1.0]
+    v0 = [1.0, 0, 0, 0, 1.0, 0]
+    qs, _ = simulate(st.model_desc(), 5.0, 0.01, q0, v0, np.zeros(6), 2)
+    return q0, v0, qs
+
+
+def check_ball_fall(q):  # joint/floating.rs:183-197
+    assert abs(q[4]) < 1e-4 and abs(q[5]) < 1e-2 and abs(q[6] - (1.0 + 5.0 - 0.5 * G * 25.0)) < 1.0
+    expect = [math.sin((math.pi / 2 + 5.0) / 2), 0.0, 0.0, math.cos((math.pi / 2 + 5.0) / 2)]
+    assert q[1] == 0.0 and q[2] == 0.0 and q[0] * q[3] * expect[0] * expect[3] > 0.0
+    assert abs(2 * math.acos(min(1.0, abs(q[3]))) - 2 * math.acos(abs(expect[3]))) < 1e-2
+
+
+def test_mass_matrix_of_a_floating_body_with_two_legs(gp, oracle):  # mechanism.rs:711
+    """8 x 8 (6 + 1 + 1), and the legs' rows couple to the floating base and
+    to each other."""
+    mb, wb, hb, ml, wl, hl = 2.0, 1.0, 0.1, 1.0, 0.1, 1.0
+    body = gp.RigidBody(gp.SpatialInertia("body", np.diag([(wb * wb + hb * hb) * mb / 12, (wb * wb + hb * hb) * mb / 12,
+                                                           (wb * wb + wb * wb) * mb / 12]), np.zeros(3), mb))
+    mlx = ml * ((wl * wl + hl * hl) / 12.0 + (hl / 2.0) ** 2)
+    leg_inertia = lambda f: gp.SpatialInertia(f, np.diag([mlx, mlx, (wl * wl + wl * wl) * ml / 12.0]),
+                                              [0.0, 0.0, -hl / 2.0 * ml], ml)
+    st = gp.MechanismState([gp.FloatingJoint(gp.Transform3D.identity("body", gp.WORLD_FRAME)),
+                            gp.RevoluteJoint(gp.Transform3D.identity("leg", "body"), [0, 1.0, 0]),
+                            gp.RevoluteJoint(gp.Transform3D.move_z("leg2", "leg", -hl), [0, 1.0, 0])],
+                           [body, gp.RigidBody(leg_inertia("leg")), gp.RigidBody(leg_inertia("leg2"))])
+    M, _ = oracle.mass_matrix(st.model_desc(), st.flat_q(), st.flat_v())
+    assert M.shape == (8, 8)
+    assert np.abs(M[6, :6]).max() > 0 and M[6, 6] != 0 and np.abs(M[7, :6]).max() > 0
+    assert M[7, 6] != 0 and M[7, 7] != 0
+    assert np.allclose(M, M.T, atol=1e-14) and np.linalg.eigvalsh(M).min() > 0
+
+
 def test_spring_forward_drop_check_fails_on_a_backward_throw(gp, oracle):
     """The bottom-of-spring check is not vacuous: thrown backwards, the body
     leans the other way and the check fails."""
