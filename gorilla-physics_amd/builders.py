@@ -240,3 +240,52 @@ def build_leg_from_foot():  # builders/leg_builder.rs:111
               RevoluteJoint(Transform3D.move_x("pelvis_left", "hip_left", -L2), [0.0, -1.0, 0.0]),
               RevoluteJoint(Transform3D.move_z("base", "pelvis_left", L2), [0.0, 0.0, 1.0])]
     return MechanismState(joints, [foot, calf, thigh, hip, pelvis, base])
+
+
+# SO-101 links (builders/mod.rs:19-249, from the arm's URDF): mass, centre of
+# mass, and the inertia about it as (ixx, ixy, ixz, iyy, iyz, izz)
+_SO101_LINKS = (
+    ("base", 0.147, (0.0137179, -5.19711e-05, 0.0334843),
+     (0.000114686, -4.59787e-07, 4.97151e-06, 0.000136117, 9.75275e-08, 0.000130364)),
+    ("shoulder", 0.100006, (-0.0307604, -1.66727e-05, -0.0252713),
+     (8.3759e-05, 7.55525e-08, -1.16342e-06, 8.10403e-05, 1.54663e-07, 2.39783e-05)),
+    ("upper_arm", 0.103, (-0.0898471, -0.00838224, 0.0184089),
+     (4.08002e-05, -1.97819e-05, -4.03016e-08, 0.000147318, 8.97326e-09, 0.000142487)),
+    ("lower_arm", 0.104, (-0.0980701, 0.00324376, 0.0182831),
+     (2.87438e-05, 7.41152e-06, 1.26409e-06, 0.000159844, -4.90188e-08, 0.00014529)),
+    ("wrist", 0.079, (-0.000103312, -0.0386143, 0.0281156),
+     (3.68263e-05, 1.7893e-08, -5.28128e-08, 2.5391e-05, 3.6412e-06, 2.1e-05)),
+    ("gripper", 0.087, (0.000213627, 0.000245138, -0.025187),
+     (2.75087e-05, -3.35241e-07, -5.7352e-06, 4.33657e-05, -5.17847e-08, 3.45059e-05)),
+    ("jaw", 0.012, (-0.00157495, -0.0300244, 0.0192755),
+     (6.61427e-06, -3.19807e-07, -5.90717e-09, 1.89032e-06, -1.09945e-07, 5.28738e-06)),
+)
+# each revolute joint (about the child's z) as xyz and roll-pitch-yaw of the
+# child frame in its parent's (builders/mod.rs:266-320)
+_SO101_JOINTS = (
+    ((0.0388353, -8.97657e-09, 0.0624), (3.14159, 4.18253e-17, -3.14159)),
+    ((-0.0303992, -0.0182778, -0.054), (-1.5708, -1.5708, 0.0)),
+    ((-0.11257, -0.028, 1.73763e-16), (-3.63608e-16, 8.74301e-16, 1.5708)),
+    ((-0.1349, 0.0052, 3.62355e-17), (4.02456e-15, 8.67362e-16, -1.5708)),
+    ((5.55112e-17, -0.0611, 0.0181), (1.5708, 0.0486795, 3.14159)),
+    ((0.0202, 0.0188, -0.0234), (1.5708, -5.24284e-08, -1.41553e-15)),
+)
+
+
+def build_so101(*meshes):  # builders/mod.rs:252
+    """The SO-101 arm: a base welded to the world and six revolute joints
+    about z (shoulder pan, shoulder lift, elbow, wrist flex, wrist roll,
+    jaw). The reference passes each link's mesh, which only draws it (its
+    collider is disabled), so any mesh arguments are accepted and unused."""
+    from .mechanism import FixedJoint, UnitQuaternion
+    bodies = []
+    for frame, m, com, (ixx, ixy, ixz, iyy, iyz, izz) in _SO101_LINKS:
+        c = np.asarray(com, dtype=float)
+        j_com = np.array([[ixx, ixy, ixz], [ixy, iyy, iyz], [ixz, iyz, izz]])
+        bodies.append(RigidBody(SpatialInertia(frame, j_com + m * (c.dot(c) * np.eye(3) - np.outer(c, c)), m * c, m)))
+    names = [link[0] for link in _SO101_LINKS]
+    joints = [FixedJoint(Transform3D.identity("base", WORLD_FRAME))]
+    for k, (xyz, rpy) in enumerate(_SO101_JOINTS):
+        iso = Isometry3(UnitQuaternion.from_euler_angles(*rpy), xyz)
+        joints.append(RevoluteJoint(Transform3D.new(names[k + 1], names[k], iso), [0.0, 0.0, 1.0]))
+    return MechanismState(joints, bodies)

@@ -164,3 +164,36 @@ def test_ball_fall(gp, oracle):  # joint/floating.rs:133
     check_ball_fall(qs[-1])
     _, _, rq = ball_fall(gp, oracle.simulate)
     assert qs.shape == rq.shape and np.abs(qs - rq).max() < 1e-12
+
+
+@pytest.mark.parametrize("integrator", [0, 2, 3])
+def test_so101_arm_step_parity(gp, oracle, integrator):  # builders/mod.rs:252
+    """64 SO-101 arms with random configurations, velocities and held
+    torques, every step against the oracle (velocity stepping at its 1e-5,
+    the others at 1e-10)."""
+    from gorilla_physics_amd import batch, builders
+    print(f"case SO-101 arms, integrator {integrator}", flush=True)
+    d = builders.build_so101().model_desc()
+    rng = np.random.default_rng(12)
+    n = 64
+    q, v, tau = rng.uniform(-2, 2, 6 * n), rng.normal(size=6 * n), rng.normal(size=6 * n) * 0.02
+    if integrator == 3:
+        s = Stepper(oracle, [d] * n, q, v, tau, integrator=VS)
+        for _ in range(50):
+            s.step(1e-3)
+        print(f"worst per-step difference {s.worst:.3g}", flush=True)
+        assert not s.b.faults().any()
+        return
+    b = batch.Batch([batch.Model(d)], [n], precision=0)
+    b.set_state(q, v)
+    b.set_tau(tau)
+    worst = 0.0
+    for k in range(100):
+        b.step(1e-3, integrator, 1)
+        q1, v1 = b.get_state()
+        a1 = b.get_vdot()
+        eq, ev, ea = step_errors(oracle, [d] * n, q, v, tau, q1, v1, a1, 1e-3, integrator)
+        worst = max(worst, eq.max(), ev.max(), ea.max())
+        q, v = q1, v1
+    print(f"worst per-step difference {worst:.3g}", flush=True)
+    assert not b.faults().any() and worst <= F64_TOL

@@ -618,6 +618,24 @@ def test_mass_matrix_of_a_floating_body_with_two_legs(gp, oracle):  # mechanism.
     assert np.allclose(M, M.T, atol=1e-14) and np.linalg.eigvalsh(M).min() > 0
 
 
+@pytest.mark.parametrize("method", METHODS)
+def test_so101_arm(gp, oracle, method):  # builders/mod.rs:252
+    """The SO-101 arm: welded base and six revolutes (nq = nv = 6), mass
+    0.632006 kg; holding it with the torques that cancel gravity (the bias
+    term at zero velocity) leaves it at rest, in any configuration."""
+    from gorilla_physics_amd import builders
+    st = builders.build_so101()
+    d = st.model_desc()
+    assert (d.n_bodies, d.nq, d.nv) == (7, 6, 6)
+    assert abs(sum(b.inertia.mass for b in st.bodies) - 0.632006) < 1e-12
+    rng = np.random.default_rng(9)
+    for _ in range(10):
+        q = rng.uniform(-2, 2, 6)
+        _, c = oracle.mass_matrix(d, q, np.zeros(6))
+        a = oracle.dynamics(d, q, np.zeros(6), c, method)
+        assert np.abs(a).max() < 1e-9 and np.abs(c).max() > 0.01
+
+
 def test_spring_forward_drop_check_fails_on_a_backward_throw(gp, oracle):
     """The bottom-of-spring check is not vacuous: thrown backwards, the body
     leans the other way and the check fails."""
