@@ -294,8 +294,9 @@ def main():
         args.steps = int(w["steps"])
     single = args.workload in ("arms", "acrobot")
     vstep = w.get("integrator", 0) >= abi.GP_VELOCITY_STEPPING
-    if vstep:  # one step per launch on the runtime-topology kernel: nothing to tune or compare
+    if vstep:  # the runtime-topology kernel in FP64: nothing to tune or compare; every step in one launch
         args.autotune, args.no_compare = 0, True
+        args.steps_per_launch = args.steps_per_launch or args.steps
     if args.autotune < 0:
         args.autotune = 0 if single else 1
     if single and not args.autotune:

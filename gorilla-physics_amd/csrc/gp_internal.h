@@ -172,7 +172,7 @@ int launch_fast_rk_step(int topo_id, int precision, int integrator, const Bucket
 size_t plain_scratch_doubles(int nb, int nq, int nv, int Bp);
 // velocity stepping (ccd = 1: with continuous collision detection), FP64, one
 // step per launch; scratch as vs_layout
-int launch_vs_step(const BucketArgs& a, double dt, int ccd, void* stream);
+int launch_vs_step(const BucketArgs& a, double dt, int ccd, int n_steps, void* stream);
 // kinetic and gravitational energy per lane (ke[lane], pe[lane])
 int launch_energy(const BucketArgs& a, double* ke, double* pe, void* stream);
 size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp);

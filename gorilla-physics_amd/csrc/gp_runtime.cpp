@@ -535,8 +535,10 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
       int st = bucket_ensure_scratch(b);
       if (st) return st;
       BucketArgs a = bucket_args(b, 1, nullptr);
-      for (int k = 0; k < n_steps; ++k) {
-        int e = launch_vs_step(a, dt, integrator == GP_CCD_VELOCITY_STEPPING ? 1 : 0, s);
+      const int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
+      for (int done = 0; done < n_steps; done += spl) {
+        const int n = std::min(spl, n_steps - done);
+        int e = launch_vs_step(a, dt, integrator == GP_CCD_VELOCITY_STEPPING ? 1 : 0, n, s);
         if (e) return fail(GP_ERR_DEVICE, "velocity-stepping launch: %s", hipGetErrorString((hipError_t)e));
         ++*launches;
       }

@@ -575,46 +575,53 @@ __device__ double earliest_collision(const VsLane& V, int nc, int* hit_body, int
 
 }  // namespace
 
-__global__ void __launch_bounds__(64) vs_step_kernel(BucketArgs a, double dt, int ccd) {
+// n_steps steps of each lane in one launch: a lane goes on to its next step
+// as soon as its own step is done, so a launch lasts as long as the slowest
+// lane's sum of steps rather than the sum of each step's slowest lane. Every
+// step reads q and v back from the state arrays it wrote, and the lane's
+// arithmetic is the same as in n_steps launches of one step.
+__global__ void __launch_bounds__(64) vs_step_kernel(BucketArgs a, double dt, int ccd, int n_steps) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= a.B) return;
   const int Bp = a.Bp, nv = a.nv, nq = a.nq;
   const VsLane V{a, lane, vs_layout(a.nb, a.nq, a.nv, a.nc, a.nh, a.ns), a.scratch + lane};
   Strided q{a.q + lane, Bp}, v{a.v + lane, Bp}, tau{const_cast<double*>(a.tau) + lane, Bp};
   Strided vdot{a.vdot + lane, Bp};
-  Strided qc = V.col(V.L.qc), vc = V.col(V.L.vc), vn = V.col(V.L.vn), zero = V.col(V.L.zero);
-  for (int k = 0; k < nv; ++k) zero[k] = 0.0;
-  for (int k = 0; k < nq; ++k) qc[k] = q[k];
-  for (int k = 0; k < nv; ++k) vc[k] = v[k];
-  bool ok = true, done = false;
-  double t_remain = dt;
-  int forced_body = -1, forced_h = -1, rounds = 0;
-  for (int round = 0; round < kCcdMaxRounds && !done; ++round) {
-    int nc = 0, hit_body = -1, hit_h = -1;
-    rounds = round + 1;
-    ok = vs_dynamics(V, qc, vc, tau, t_remain, &nc, forced_body, forced_h) && ok;
-    const double earliest = ccd ? earliest_collision(V, nc, &hit_body, &hit_h) : 1e300;
-    if (earliest < t_remain) {
-      plain_advance(a, lane, qc, vn, earliest, qc);
-      for (int k = 0; k < nv; ++k) vc[k] = vn[k];
-      t_remain -= earliest;
-      forced_body = hit_body;
-      forced_h = hit_body >= 0 ? hit_h : -1;
-    } else {
-      plain_advance(a, lane, qc, vn, t_remain, qc);
-      done = true;
+  for (int step = 0; step < n_steps; ++step) {
+    Strided qc = V.col(V.L.qc), vc = V.col(V.L.vc), vn = V.col(V.L.vn), zero = V.col(V.L.zero);
+    for (int k = 0; k < nv; ++k) zero[k] = 0.0;
+    for (int k = 0; k < nq; ++k) qc[k] = q[k];
+    for (int k = 0; k < nv; ++k) vc[k] = v[k];
+    bool ok = true, done = false;
+    double t_remain = dt;
+    int forced_body = -1, forced_h = -1, rounds = 0;
+    for (int round = 0; round < kCcdMaxRounds && !done; ++round) {
+      int nc = 0, hit_body = -1, hit_h = -1;
+      rounds = round + 1;
+      ok = vs_dynamics(V, qc, vc, tau, t_remain, &nc, forced_body, forced_h) && ok;
+      const double earliest = ccd ? earliest_collision(V, nc, &hit_body, &hit_h) : 1e300;
+      if (earliest < t_remain) {
+        plain_advance(a, lane, qc, vn, earliest, qc);
+        for (int k = 0; k < nv; ++k) vc[k] = vn[k];
+        t_remain -= earliest;
+        forced_body = hit_body;
+        forced_h = hit_body >= 0 ? hit_h : -1;
+      } else {
+        plain_advance(a, lane, qc, vn, t_remain, qc);
+        done = true;
+      }
     }
+    ok = ok && done;  // the collision loop gave up: a fault, the state still advances
+    if (!done) plain_advance(a, lane, qc, vn, t_remain, qc);
+    for (int k = 0; k < nv; ++k) {
+      vdot[k] = (vn[k] - v[k]) / dt;
+      v[k] = vn[k];
+    }
+    for (int k = 0; k < nq; ++k) q[k] = qc[k];
+    ok = ok && finite_cols(q, nq) && finite_cols(v, nv);
+    if (!ok) a.fault[lane] = 1;
+    a.rounds[lane] = rounds;
   }
-  ok = ok && done;  // the collision loop gave up: a fault, the state still advances
-  if (!done) plain_advance(a, lane, qc, vn, t_remain, qc);
-  for (int k = 0; k < nv; ++k) {
-    vdot[k] = (vn[k] - v[k]) / dt;
-    v[k] = vn[k];
-  }
-  for (int k = 0; k < nq; ++k) q[k] = qc[k];
-  ok = ok && finite_cols(q, nq) && finite_cols(v, nv);
-  if (!ok) a.fault[lane] = 1;
-  a.rounds[lane] = rounds;
 }
 
 size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp) {
@@ -624,10 +631,10 @@ size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp
 // 64-lane workgroups: one instance per lane is a long serial program, so the
 // launch spreads its waves over every compute unit rather than packing four
 // to a workgroup on a quarter of them
-int launch_vs_step(const BucketArgs& a, double dt, int ccd, void* stream) {
-  if (a.B == 0) return 0;
+int launch_vs_step(const BucketArgs& a, double dt, int ccd, int n_steps, void* stream) {
+  if (a.B == 0 || n_steps <= 0) return 0;
   dim3 block(64), grid((a.B + 63) / 64);
-  hipLaunchKernelGGL(vs_step_kernel, grid, block, 0, (hipStream_t)stream, a, dt, ccd);
+  hipLaunchKernelGGL(vs_step_kernel, grid, block, 0, (hipStream_t)stream, a, dt, ccd, n_steps);
   return (int)hipGetLastError();
 }
 

@@ -168,7 +168,8 @@ typedef struct gp_model_desc {
 typedef struct gp_batch_opts {
   int32_t precision;         /* GP_PREC_*                                     */
   int32_t kernel;            /* GP_KERNEL_*                                   */
-  int32_t steps_per_launch;  /* FAST kernels: steps fused in one launch (0 = 1) */
+  int32_t steps_per_launch;  /* steps fused in one launch by the FAST and
+                                velocity-stepping kernels (0 = 100)           */
   int32_t block_size;        /* threads per workgroup (0 = 256)               */
   int32_t lanes_per_mech;    /* FAST free bodies with contacts: lanes sharing
                                 one instance's contact points, 1/2/4/8 (0 = 4) */

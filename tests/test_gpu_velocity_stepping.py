@@ -269,6 +269,25 @@ def test_ccd_cubes_as_one_batch(gp, oracle):  # collision/ccd/mod.rs:229,284
     assert not s.b.faults().any()
 
 
+def test_steps_in_one_launch_are_bit_identical(gp):
+    """Several steps in one launch (steps_per_launch 7, so the last launch
+    is partial) give the same states, fault flags and collision-loop rounds,
+    bit for bit, as one launch per step: 200 thrown cubes, 30 CCD steps."""
+    from gorilla_physics_amd import batch, workloads
+    print("case steps per launch 1 vs 7, 200 thrown cubes", flush=True)
+    w = workloads.thrown_cubes(n=200)
+    out = []
+    for spl in (1, 7):
+        b = batch.Batch([batch.Model(w["models"][0])], [200], precision=0, steps_per_launch=spl)
+        b.set_state(w["q"], w["v"])
+        b.step(w["dt"], CCD, 30)
+        _, launches = b.last_timing()
+        q, v = b.get_state()
+        out.append((np.concatenate([q, v]), b.faults(), b.collision_rounds(), launches))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2]) and (out[0][3], out[1][3]) == (30, 5)
+
+
 def test_velocity_stepping_rejects_mixed_plain(gp):
     from gorilla_physics_amd import batch, builders
     st = builders.build_cube(1.0, 1.0)
