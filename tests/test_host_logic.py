@@ -129,3 +129,21 @@ def test_interface_biped_and_leg_poses(gp):
     assert np.abs(st.poses()[-1].translation).max() < 1e-15
     st = interface.create_leg_from_foot()
     assert np.abs(st.poses()[0].translation).max() == 0.0 and st.q[1] == math.pi / 4 - 0.4
+
+
+def test_bodies_to_root_of_a_branched_tree(gp, oracle):  # spatial/transform.rs:242
+    """Bodies 2 and 4 at x = -1 and +1 beside body 1, bodies 3 and 5 one
+    above each: at zero angles body 2 sits at (-1, 0, 0) and body 5 at
+    (1, 0, 1) in the world; the oracle's gravitational energy (m g z of each
+    frame origin) sees the two raised bodies."""
+    T = gp.Transform3D
+    joints = [gp.FloatingJoint(T.identity("1", gp.WORLD_FRAME)),
+              gp.RevoluteJoint(T.move_x("2", "1", -1.0), [0, 1.0, 0]),
+              gp.RevoluteJoint(T.move_z("3", "2", 1.0), [0, 1.0, 0]),
+              gp.RevoluteJoint(T.move_x("4", "1", 1.0), [0, 1.0, 0]),
+              gp.RevoluteJoint(T.move_z("5", "4", 1.0), [0, 1.0, 0])]
+    st = gp.MechanismState(joints, [gp.RigidBody.new_sphere(1.0, 1.0, f) for f in "12345"])
+    p = st.poses()
+    assert np.array_equal(p[0].translation, [0, 0, 0]) and p[0].rotation.coords() == [0, 0, 0, 1.0]
+    assert np.array_equal(p[1].translation, [-1.0, 0, 0]) and np.array_equal(p[4].translation, [1.0, 0, 1.0])
+    assert abs(oracle.gravitational_energy(st.model_desc(), st.flat_q()) - 2 * 9.81) < 1e-12
