@@ -317,7 +317,7 @@ def main():
         log("fast f64", f64)
         # the same steps (for the cubes, the same free-fall and contact phases)
         plain = run_variant(w, models, args.steps, min(args.warmup, 10), precision=abi.GP_PREC_F64,
-                            kernel=abi.GP_KERNEL_PLAIN, use_graph=args.graph)  # one step per launch: nothing to tune
+                            kernel=abi.GP_KERNEL_PLAIN, use_graph=args.graph)  # runtime topology: nothing to tune
         log("plain f64", plain)
         extra = dict(fast_f64_env_steps_per_s=f64["value"], plain_f64_env_steps_per_s=plain["value"],
                      speedup_mixed_vs_plain_f64=mixed["value"] / plain["value"],

@@ -160,7 +160,7 @@ inline int fast_lanes(int fast_topo, int lanes_per_mech, int lanes_per_tree) {
 // Host-side launchers (gp_plain.hip, gp_fast.hip). Each returns a hipError_t as int.
 int launch_grouped_step(const GroupTable* d_table, int n_blocks, int precision, double dt, int n_steps, int block,
                         void* stream);
-int launch_plain_step(const BucketArgs& a, double dt, int integrator, int dynamics_only, void* stream);
+int launch_plain_step(const BucketArgs& a, double dt, int integrator, int dynamics_only, int n_steps, void* stream);
 // Fast kernels: topo_id from fast_topology_id(); precision GP_PREC_*.
 int fast_topology_id(int nb, const int* parent, const int* dofcls, int nc);
 int launch_fast_step(int topo_id, int precision, const BucketArgs& a, double dt, int n_steps, int block,

@@ -562,8 +562,9 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
       int st = bucket_ensure_scratch(b);
       if (st) return st;
       BucketArgs a = bucket_args(b, 1, nullptr);
-      for (int k = 0; k < n_steps; ++k) {
-        int e = launch_plain_step(a, dt, integrator, 0, s);
+      const int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
+      for (int done = 0; done < n_steps; done += spl) {
+        int e = launch_plain_step(a, dt, integrator, 0, std::min(spl, n_steps - done), s);
         if (e) return fail(GP_ERR_DEVICE, "plain kernel launch: %s", hipGetErrorString((hipError_t)e));
         ++*launches;
       }
@@ -958,7 +959,7 @@ int gp_dynamics(gp_batch* bt, const double* tau, double* vdot) {
   for (Bucket& b : bt->buckets) {
     if ((st = bucket_ensure_scratch(b))) return st;
     BucketArgs a = bucket_args(b, 1, tau ? b.d_tau_tmp : nullptr);
-    int e = launch_plain_step(a, 0.0, GP_SEMI_IMPLICIT_EULER, 1, bt->stream);
+    int e = launch_plain_step(a, 0.0, GP_SEMI_IMPLICIT_EULER, 1, 1, bt->stream);
     if (e) return fail(GP_ERR_DEVICE, "dynamics launch: %s", hipGetErrorString((hipError_t)e));
   }
   return transfer(bt, nullptr, nullptr, nullptr, vdot, false, 2);

@@ -83,8 +83,8 @@ enum {
 /* ---- kernel family --------------------------------------------------------- */
 enum {
   GP_KERNEL_AUTO = 0,  /* FAST when the bucket topology has a compiled kernel */
-  GP_KERNEL_PLAIN = 1, /* runtime-topology FP64 kernel, one step per launch,
-                          per-body intermediates in device scratch            */
+  GP_KERNEL_PLAIN = 1, /* runtime-topology FP64 kernel, per-body
+                          intermediates in device scratch                     */
   GP_KERNEL_FAST = 2   /* compile-time topology, register resident, several
                           steps per launch                                     */
 };
@@ -168,8 +168,8 @@ typedef struct gp_model_desc {
 typedef struct gp_batch_opts {
   int32_t precision;         /* GP_PREC_*                                     */
   int32_t kernel;            /* GP_KERNEL_*                                   */
-  int32_t steps_per_launch;  /* steps fused in one launch by the FAST and
-                                velocity-stepping kernels (0 = 100)           */
+  int32_t steps_per_launch;  /* steps fused in one launch by every kernel
+                                (0 = 100)                                     */
   int32_t block_size;        /* threads per workgroup (0 = 256)               */
   int32_t lanes_per_mech;    /* FAST free bodies with contacts: lanes sharing
                                 one instance's contact points, 1/2/4/8 (0 = 4) */

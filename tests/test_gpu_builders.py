@@ -197,3 +197,31 @@ def test_so101_arm_step_parity(gp, oracle, integrator):  # builders/mod.rs:252
         q, v = q1, v1
     print(f"worst per-step difference {worst:.3g}", flush=True)
     assert not b.faults().any() and worst <= F64_TOL
+
+
+def test_plain_kernel_steps_in_one_launch_are_bit_identical(gp):
+    """The runtime-topology kernel with 7 steps per launch (the last launch
+    partial) against one launch per step, 30 steps, bit for bit: SLIP
+    hoppers (per-instance spring-contact state, SIE), SO-101 arms (RK4),
+    welded arms (RK2)."""
+    from gorilla_physics_amd import batch, builders
+    from test_oracle_golden import welded_arm
+    rng = np.random.default_rng(21)
+    slip = builders.build_SLIP(0.54, 0.1, 0.2, 0.3, 2e3)
+    slip.add_halfspace(gp.HalfSpace.new([0, 0, 1.0], 0.0))
+    cases = [(slip.model_desc(), 0, np.tile([0, 0, 0, 1.0, 0, 0, 0.25], 16),
+              np.concatenate([[0, 0, 0, u, 0, -1.0] for u in rng.uniform(0.5, 2.0, 16)])),
+             (builders.build_so101().model_desc(), 2, rng.uniform(-2, 2, 6 * 16), rng.normal(size=6 * 16)),
+             (welded_arm(gp).model_desc(), 1, rng.uniform(-2, 2, 2 * 16), rng.normal(size=2 * 16))]
+    for d, integrator, q0, v0 in cases:
+        print(f"case plain kernel steps per launch, integrator {integrator}, nq {d.nq}", flush=True)
+        out = []
+        for spl in (1, 7):
+            b = batch.Batch([batch.Model(d)], [16], precision=0, kernel=1, steps_per_launch=spl)
+            b.set_state(q0, v0)
+            b.step(1e-3, integrator, 30)
+            _, launches = b.last_timing()
+            q, v = b.get_state()
+            out.append((np.concatenate([q, v, b.get_vdot()]), b.faults(), launches))
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+        assert (out[0][2], out[1][2]) == (30, 5)
