@@ -56,6 +56,8 @@ def make_workload(name, n):
         return workloads.acrobot(n=n or 1024)
     if name == "thrown_cubes":
         return workloads.thrown_cubes(n=n or 16384)
+    if name == "so101":
+        return workloads.so101(n=n or 65536)
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -293,7 +295,8 @@ def main():
     if args.steps <= 0:
         args.steps = int(w["steps"])
     single = args.workload in ("arms", "acrobot")
-    vstep = w.get("integrator", 0) >= abi.GP_VELOCITY_STEPPING
+    is_vs = w.get("integrator", 0) >= abi.GP_VELOCITY_STEPPING
+    vstep = is_vs or args.workload == "so101"
     if vstep:  # the runtime-topology kernel in FP64: nothing to tune or compare; every step in one launch
         args.autotune, args.no_compare = 0, True
         args.steps_per_launch = args.steps_per_launch or args.steps
@@ -358,11 +361,15 @@ def main():
             "cubes": "16,384 free cubes x 8 corner contacts on z=0, dt=1e-4, semi-implicit Euler, mixed precision",
             "acrobot": "1,024 acrobots, dt=1e-3, semi-implicit Euler, mixed precision",
             "thrown_cubes": "16,384 cubes (side 0.1) thrown at z=0, dt=1/120, velocity stepping with continuous "
-                            "collision detection (cone-program contact impulses), FP64"}[args.workload],
+                            "collision detection (cone-program contact impulses), FP64",
+            "so101": "65,536 SO-101 arms (welded base, 6 revolutes), dt=1e-3, RK4, held torques, "
+                     "runtime-topology FP64 kernel"}[args.workload],
             "instances": mixed["instances"], "steps_per_launch": mixed["opts"]["steps_per_launch"] or spl,
             "hip_graph": bool(args.graph), "launch_opts": mixed["opts"], "autotuned": bool(args.autotune),
             "error_bound": ("teacher-forced per-step vs FP64 oracle: e(q'), e(v') <= 1e-5 (contact impulses from an "
-                            "interior-point method), tests/test_gpu_velocity_stepping.py") if vstep else
+                            "interior-point method), tests/test_gpu_velocity_stepping.py") if is_vs else
+                           ("teacher-forced per-step vs FP64 oracle: e(q'), e(v'), e(qdd) <= 1e-10, "
+                            "tests/test_gpu_builders.py") if vstep else
                            ("teacher-forced per-step vs FP64 oracle: e(q') <= 1e-5, e(v'), e(qdd) <= 2e-3, "
                             "p99 e(qdd) <= 1e-4 (FP64 kernels: all <= 1e-10), tests/test_gpu_parity.py")},
         "roofline": roof,

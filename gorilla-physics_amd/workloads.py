@@ -21,7 +21,7 @@ from . import abi
 from .mechanism import (GRAVITY, Isometry3, ModelDesc, SpatialInertia, UnitQuaternion, body_desc, contact_desc,
                         halfspace_desc, HalfSpace)
 
-WORKLOAD_IDS = {"acrobot": 1, "arms": 2, "mixed": 3, "cubes": 4, "thrown_cubes": 5}
+WORKLOAD_IDS = {"acrobot": 1, "arms": 2, "mixed": 3, "cubes": 4, "thrown_cubes": 5, "so101": 6}
 
 
 def stream(seed, workload, field):
@@ -158,6 +158,20 @@ def cube_desc(k_point=12500.0, alpha=1.0 / 150.0, mu=0.5, m=1.0, l=1.0):
     contacts = [contact_desc(0, p, k_point) for p in corners]
     hs = [halfspace_desc(HalfSpace.new_with_params([0.0, 0.0, 1.0], 0.0, alpha, mu))]
     return ModelDesc(bodies, contacts, hs)
+
+
+def so101(n=65536, seed=0):
+    """SO-101 arms (builders/mod.rs:252: welded base, six revolutes, the
+    URDF's inertias): joint angles U(-pi, pi), velocities N(0, 0.5), held
+    torques N(0, 0.02) N m, RK4 at dt 1e-3 for 1000 steps. The arm has no
+    compiled topology (its root joint is fixed), so it runs on the
+    runtime-topology FP64 kernel."""
+    from . import builders
+    d = builders.build_so101().model_desc()
+    q = stream(seed, "so101", 1).uniform(-np.pi, np.pi, 6 * n)
+    v = stream(seed, "so101", 2).normal(0.0, 0.5, 6 * n)
+    tau = stream(seed, "so101", 3).normal(0.0, 0.02, 6 * n)
+    return dict(models=[d], counts=[n], q=q, v=v, tau=tau, dt=1e-3, steps=1000, name="so101", integrator=2)
 
 
 def thrown_cubes(n=16384, seed=0):
