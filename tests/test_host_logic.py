@@ -147,3 +147,13 @@ def test_bodies_to_root_of_a_branched_tree(gp, oracle):  # spatial/transform.rs:
     assert np.array_equal(p[0].translation, [0, 0, 0]) and p[0].rotation.coords() == [0, 0, 0, 1.0]
     assert np.array_equal(p[1].translation, [-1.0, 0, 0]) and np.array_equal(p[4].translation, [1.0, 0, 1.0])
     assert abs(oracle.gravitational_energy(st.model_desc(), st.flat_q()) - 2 * 9.81) < 1e-12
+
+
+def test_so101_workload_shape_and_prefix(gp):
+    """One model, six coordinates per arm, and instance i's state and
+    torques do not depend on the batch size."""
+    from gorilla_physics_amd import workloads
+    a, b = workloads.so101(n=5), workloads.so101(n=9)
+    assert a["counts"] == [5] and a["models"][0].nq == 6 and a["q"].size == 30 and a["integrator"] == 2
+    for key in ("q", "v", "tau"):
+        assert np.array_equal(a[key], b[key][:30])
