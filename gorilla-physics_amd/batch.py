@@ -113,8 +113,31 @@ class Batch:
     def step(self, dt, integrator=abi.GP_SEMI_IMPLICIT_EULER, n_steps=1, stream=None):
         abi.check(self.lib.gp_step(self.handle, float(dt), int(integrator), int(n_steps), stream), self.lib)
 
+    def prepare(self, dt, integrator=abi.GP_SEMI_IMPLICIT_EULER, n_steps=1, stream=None):
+        """gp_step_prepare: build what step(dt, integrator, n_steps) launches
+        (group table, scratch, graphs) without stepping."""
+        abi.check(self.lib.gp_step_prepare(self.handle, float(dt), int(integrator), int(n_steps), stream), self.lib)
+
     def sync(self):
         abi.check(self.lib.gp_batch_sync(self.handle), self.lib)
+
+    def mass_matrices(self):
+        """gp_mass_matrix: M(q) of every instance (a list of nv_i x nv_i arrays)."""
+        _, vo = self.offsets()
+        sizes = [int(n) for n in np.diff(vo)]
+        out = np.empty(max(1, sum(n * n for n in sizes)))
+        abi.check(self.lib.gp_mass_matrix(self.handle, _dp(out)), self.lib)
+        res, o = [], 0
+        for n in sizes:
+            res.append(out[o:o + n * n].reshape(n, n).copy())
+            o += n * n
+        return res
+
+    def dynamics_bias(self):
+        """gp_dynamics_bias: c(q, v) - tau_c of every instance, flattened like v."""
+        out = np.empty(max(1, self.nv_total))
+        abi.check(self.lib.gp_dynamics_bias(self.handle, _dp(out)), self.lib)
+        return out[:self.nv_total]
 
     def dynamics(self, tau=None):
         out = np.empty(self.nv_total)

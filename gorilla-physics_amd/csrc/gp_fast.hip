@@ -19,10 +19,15 @@ int fast_topology_id(int nb, const int* parent, const int* dofcls, int nc) {
   bool chain = nb >= 1 && nb <= 12 && nc == 0;
   for (int i = 0; i < nb && chain; ++i) chain = parent[i] == i - 1 && dofcls[i] == 1;
   if (chain) return nb;
+  // a welded base and a chain behind it
+  bool welded = nb >= 2 && nb - 1 <= kMaxWeldedJoints && nc == 0 && parent[0] == -1 && dofcls[0] == 0;
+  for (int i = 1; i < nb && welded; ++i) welded = parent[i] == i - 1 && dofcls[i] == 1;
+  if (welded) return TOPO_WELDED + (nb - 1);
   if (nb == 1 && parent[0] == -1 && dofcls[0] == 6) return TOPO_FREE;
   bool quad = nb == 13 && nc == 0;
   for (int i = 0; i < nb && quad; ++i) quad = parent[i] == TopoQuad::parent(i) && dofcls[i] == TopoQuad::dof(i);
   if (quad) return TOPO_QUAD;
+  if (tree_topology_ok(nb, parent)) return TOPO_TREE;
   return -1;
 }
 

@@ -34,22 +34,40 @@ namespace gp {
 template <int N_>
 struct TopoChain {
   static constexpr int N = N_;
+  static constexpr int first = 0;  // first body the sweeps visit
   static constexpr __host__ __device__ int parent(int i) { return i - 1; }
   static constexpr __host__ __device__ int dof(int) { return 1; }
   static constexpr __host__ __device__ int qdim(int) { return 1; }
 };
 struct TopoFree {
   static constexpr int N = 1;
+  static constexpr int first = 0;
   static constexpr __host__ __device__ int parent(int) { return -1; }
   static constexpr __host__ __device__ int dof(int) { return 6; }
   static constexpr __host__ __device__ int qdim(int) { return 7; }
 };
 struct TopoQuad {  // floating base + 4 legs of 3 one-DOF joints
   static constexpr int N = 13;
+  static constexpr int first = 0;
   static constexpr __host__ __device__ int parent(int i) { return i == 0 ? -1 : ((i - 1) % 3 == 0 ? 0 : i - 1); }
   static constexpr __host__ __device__ int dof(int i) { return i == 0 ? 6 : 1; }
   static constexpr __host__ __device__ int qdim(int i) { return i == 0 ? 7 : 1; }
 };
+// A body welded to the world (a fixed joint, no coordinates) and a chain of
+// N - 1 one-DOF joints behind it: a fixed-base manipulator (the SO-101 of
+// builders/mod.rs:252). Body 0 never moves, so the sweeps start at body 1 as
+// the root of a chain whose base acceleration is gravity seen in body 0's
+// frame, formed once per launch.
+template <int N_>
+struct TopoWelded {
+  static constexpr int N = N_;
+  static constexpr int first = 1;
+  static constexpr __host__ __device__ int parent(int i) { return i - 1; }
+  static constexpr __host__ __device__ int dof(int i) { return i == 0 ? 0 : 1; }
+  static constexpr __host__ __device__ int qdim(int i) { return i == 0 ? 0 : 1; }
+};
+// the parent a sweep sees: the first swept body is a root
+template <class T> constexpr __host__ __device__ int sweep_parent(int i) { return i == T::first ? -1 : T::parent(i); }
 template <class T> constexpr __host__ __device__ int qoff_of(int i) {
   int s = 0;
   for (int k = 0; k < i; ++k) s += T::qdim(k);
@@ -204,7 +222,7 @@ struct FastParams {
 template <class Topo, class R, bool ALLREV> constexpr int cached_bodies() {
   constexpr int n_max = (sizeof(R) == 4 ? GP_PCACHE_F32 : GP_PCACHE_F64) - (ALLREV ? 0 : 1);
   if (Topo::N == 1) return 1;
-  if (Topo::dof(0) != 1) return 0;
+  if (Topo::dof(Topo::first) != 1) return 0;
   if (Topo::N <= n_max) return Topo::N;
   return sizeof(R) == 4 ? (ALLREV ? GP_PCACHE_LONG : GP_PCACHE_LONG / 2) : 0;
 }
@@ -241,6 +259,8 @@ __device__ __forceinline__ bool inverse_inertia(const ai6<R>& I, double (&Iinv)[
 template <class Topo, int L, bool CONTACTS, bool TAU>
 struct LaunchConst {
   static constexpr bool FREE = lone_free_body<Topo>();
+  static constexpr bool WELDED = Topo::first > 0;
+  double abase[WELDED ? 3 : 1];  // welded base: linear acceleration of body 0 (gravity in its frame)
   static constexpr bool TAU_REG = TAU || FREE;
   static constexpr int MAXC = (FREE && CONTACTS) ? (8 + L - 1) / L : 1;  // points per lane
   static constexpr int MAXH = (FREE && CONTACTS) ? 2 : 1;
@@ -258,6 +278,13 @@ __device__ __forceinline__ bool load_launch_const(const BucketArgs& a, const Fas
                                                   int sub, LaunchConst<Topo, L, CONTACTS, TAU>& lc) {
   using LC = LaunchConst<Topo, L, CONTACTS, TAU>;
   bool ok = true;
+  if constexpr (LC::WELDED) {
+    // body 0 (row f * N + 0 of field f): E = R_t^T, so E (0, 0, g) = g times
+    // the third row of R_t
+    const Soa<double> bpS(a.bp, NBF * Topo::N, a.Bp);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lc.abase[k] = a.g * bpS.ld((BF_RT + 6 + k) * Topo::N, lane);
+  }
   if constexpr (LC::TAU_REG) {
     const Soa<double> tauS(a.tau, nv_of<Topo>(), a.Bp);
 #pragma unroll
@@ -320,8 +347,8 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
   bool ok = true;
   // ---- pass 1: kinematics, velocities, bias terms (root to leaves)
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const int pr = Topo::parent(i);
+  for (int i = Topo::first; i < N; ++i) {
+    const int pr = sweep_parent<Topo>(i);
     const int pp = pr < 0 ? 0 : pr;
     m3<R> E;
     v3<R> r;
@@ -440,8 +467,8 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
   for (int i = 0; i < N; ++i) has_acc[i] = false;
   double Lf[21], D6[6][6], u6[6];
 #pragma unroll
-  for (int i = N - 1; i >= 0; --i) {
-    const int pr = Topo::parent(i);
+  for (int i = N - 1; i >= Topo::first; --i) {
+    const int pr = sweep_parent<Topo>(i);
     const int pp = pr < 0 ? 0 : pr;
     ai6<R> IA = P.inertia(i);
     if (has_acc[i]) add_to(IA, acc[i]);
@@ -486,10 +513,11 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
   }
   // ---- pass 3: accelerations (root to leaves)
   sv<R> accel[N];
-  const sv<R> a0{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}};
+  sv<R> a0{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}};
+  if constexpr (LC::WELDED) a0.v = v3<R>{R(lc.abase[0]), R(lc.abase[1]), R(lc.abase[2])};
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const int pr = Topo::parent(i);
+  for (int i = Topo::first; i < N; ++i) {
+    const int pr = sweep_parent<Topo>(i);
     const int pp = pr < 0 ? 0 : pr;
     if (Topo::dof(i) == 1) {
       const Body1<R>& B = b[i];
@@ -612,7 +640,7 @@ __device__ __forceinline__ void euler_stage(const double* q, const double* v, co
 #pragma unroll
   for (int i = 0; i < Topo::N; ++i) {
     if (Topo::dof(i) == 1) qo[qoff_of<Topo>(i)] = q[qoff_of<Topo>(i)] + v[voff_of<Topo>(i)] * h;
-    else advance_floating(&q[qoff_of<Topo>(i)], &v[voff_of<Topo>(i)], h, &qo[qoff_of<Topo>(i)]);
+    else if (Topo::dof(i) == 6) advance_floating(&q[qoff_of<Topo>(i)], &v[voff_of<Topo>(i)], h, &qo[qoff_of<Topo>(i)]);
   }
 #pragma unroll
   for (int k = 0; k < nv_of<Topo>(); ++k) vo[k] = v[k] + acc[k] * h;
@@ -917,6 +945,11 @@ __global__ void __launch_bounds__(256) tree_sie_kernel(BucketArgs a, double dt, 
   tree_sie_body<R, LEGS, DEPTH, ALLREV>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps);
 }
 
+// ---------------------------------------------------------------- trees, one body per lane
+}  // namespace gp
+#include "gp_tree_impl.h"
+namespace gp {
+
 // ---------------------------------------------------------------- grouped launch
 // Every bucket of a batch in one launch: bucket b owns workgroups
 // [block_start[b], block_start[b + 1]) and each workgroup runs its bucket's
@@ -934,6 +967,14 @@ __device__ __forceinline__ void grouped_dispatch(const BucketArgs& a, int topo, 
     GP_CHAIN_CASE(1) GP_CHAIN_CASE(2) GP_CHAIN_CASE(3) GP_CHAIN_CASE(4) GP_CHAIN_CASE(5) GP_CHAIN_CASE(6)
     GP_CHAIN_CASE(7) GP_CHAIN_CASE(8) GP_CHAIN_CASE(9) GP_CHAIN_CASE(10) GP_CHAIN_CASE(11) GP_CHAIN_CASE(12)
 #undef GP_CHAIN_CASE
+#define GP_WELDED_CASE(K)                                                              \
+  case TOPO_WELDED + K:                                                                \
+    if (ar) fast_sie_body<TopoWelded<K + 1>, R, 1, false, true>(a, tid, dt, n);        \
+    else fast_sie_body<TopoWelded<K + 1>, R, 1, false, false>(a, tid, dt, n);          \
+    break;
+    GP_WELDED_CASE(1) GP_WELDED_CASE(2) GP_WELDED_CASE(3) GP_WELDED_CASE(4) GP_WELDED_CASE(5) GP_WELDED_CASE(6)
+    GP_WELDED_CASE(7)
+#undef GP_WELDED_CASE
     case TOPO_QUAD:
       if (a.lanes == 4) {
         if (ar) tree_sie_body<R, 4, 3, true>(a, tid, dt, n);
@@ -941,6 +982,14 @@ __device__ __forceinline__ void grouped_dispatch(const BucketArgs& a, int topo, 
       } else {
         if (ar) fast_sie_body<TopoQuad, R, 1, false, true>(a, tid, dt, n);
         else fast_sie_body<TopoQuad, R, 1, false, false>(a, tid, dt, n);
+      }
+      break;
+    case TOPO_TREE:
+      switch (a.lanes) {
+        case 16: wave_tree_body<R, 16>(a, tid, dt, n); break;
+        case 32: wave_tree_body<R, 32>(a, tid, dt, n); break;
+        case 64: wave_tree_body<R, 64>(a, tid, dt, n); break;
+        default: break;
       }
       break;
     case TOPO_FREE:
@@ -1011,6 +1060,11 @@ static int launch_rk_prec(int topo, const BucketArgs& a, double dt, int n, int i
     GP_RK_CASE(1) GP_RK_CASE(2) GP_RK_CASE(3) GP_RK_CASE(4) GP_RK_CASE(5) GP_RK_CASE(6)
     GP_RK_CASE(7) GP_RK_CASE(8) GP_RK_CASE(9) GP_RK_CASE(10) GP_RK_CASE(11) GP_RK_CASE(12)
 #undef GP_RK_CASE
+#define GP_RK_WELDED_CASE(K) \
+    case TOPO_WELDED + K: return launch_rk_chain<TopoWelded<K + 1>, R>(a, dt, n, integ, block, s);
+    GP_RK_WELDED_CASE(1) GP_RK_WELDED_CASE(2) GP_RK_WELDED_CASE(3) GP_RK_WELDED_CASE(4) GP_RK_WELDED_CASE(5)
+    GP_RK_WELDED_CASE(6) GP_RK_WELDED_CASE(7)
+#undef GP_RK_WELDED_CASE
     case TOPO_QUAD: return launch_rk_chain<TopoQuad, R>(a, dt, n, integ, block, s);
     case TOPO_FREE:
       switch (a.lanes) {
@@ -1038,9 +1092,17 @@ static int launch_prec(int topo, const BucketArgs& a, double dt, int n, int bloc
     case 10: return launch_chain<TopoChain<10>, R>(a, dt, n, block, s);
     case 11: return launch_chain<TopoChain<11>, R>(a, dt, n, block, s);
     case 12: return launch_chain<TopoChain<12>, R>(a, dt, n, block, s);
+    case TOPO_WELDED + 1: return launch_chain<TopoWelded<2>, R>(a, dt, n, block, s);
+    case TOPO_WELDED + 2: return launch_chain<TopoWelded<3>, R>(a, dt, n, block, s);
+    case TOPO_WELDED + 3: return launch_chain<TopoWelded<4>, R>(a, dt, n, block, s);
+    case TOPO_WELDED + 4: return launch_chain<TopoWelded<5>, R>(a, dt, n, block, s);
+    case TOPO_WELDED + 5: return launch_chain<TopoWelded<6>, R>(a, dt, n, block, s);
+    case TOPO_WELDED + 6: return launch_chain<TopoWelded<7>, R>(a, dt, n, block, s);
+    case TOPO_WELDED + 7: return launch_chain<TopoWelded<8>, R>(a, dt, n, block, s);
     case TOPO_QUAD:
       if (a.lanes == 4) return launch_tree<R, 4, 3>(a, dt, n, block, s);
       return launch_chain<TopoQuad, R>(a, dt, n, block, s);
+    case TOPO_TREE: return launch_wave_tree<R>(a, dt, n, block, s);
     case TOPO_FREE:
       switch (a.lanes) {
         case 2: return launch_t<TopoFree, R, 2, true, false>(a, dt, n, block, s);

@@ -145,15 +145,22 @@ struct GroupTable {
   BucketArgs args[kMaxGroup];
 };
 
-// Compile-time topologies besides the chains (whose id is their body count).
-enum { TOPO_FREE = 100, TOPO_QUAD = 101 };
+// Compile-time topologies besides the chains (whose id is their body count);
+// TOPO_TREE: any tree of 13 to 64 bodies, one body per lane (gp_tree_impl.h);
+// TOPO_WELDED + k: a body welded to the world and a chain of k one-DOF joints
+// behind it, k = 1..7 (a fixed-base manipulator).
+enum { TOPO_FREE = 100, TOPO_QUAD = 101, TOPO_TREE = 102, TOPO_WELDED = 200 };
+constexpr int kMaxWeldedJoints = 7;
 
 // Lanes per instance of a FAST bucket: free bodies with contacts split their
 // contact points over lanes_per_mech lanes (0 = 4); the quadruped tree runs one
 // lane per leg unless lanes_per_tree is 1.
-inline int fast_lanes(int fast_topo, int lanes_per_mech, int lanes_per_tree) {
+// A TOPO_TREE bucket runs one body per lane on 16, 32 or 64 lanes per
+// instance (the power of two that holds its bodies).
+inline int fast_lanes(int fast_topo, int lanes_per_mech, int lanes_per_tree, int nb = 0) {
   if (fast_topo == TOPO_FREE) return lanes_per_mech > 0 ? lanes_per_mech : 4;
   if (fast_topo == TOPO_QUAD) return lanes_per_tree == 1 ? 1 : 4;
+  if (fast_topo == TOPO_TREE) return nb <= 16 ? 16 : nb <= 32 ? 32 : 64;
   return 1;
 }
 
@@ -173,6 +180,9 @@ size_t plain_scratch_doubles(int nb, int nq, int nv, int Bp);
 // velocity stepping (ccd = 1: with continuous collision detection), FP64, one
 // step per launch; scratch as vs_layout
 int launch_vs_step(const BucketArgs& a, double dt, int ccd, int n_steps, void* stream);
+// mass matrix and bias c(q, v) - tau_c per lane: out is [nv * nv + nv][Bp]
+// (M row-major, then c); needs the bucket's plain scratch
+int launch_terms(const BucketArgs& a, double* out, void* stream);
 // kinetic and gravitational energy per lane (ke[lane], pe[lane])
 int launch_energy(const BucketArgs& a, double* ke, double* pe, void* stream);
 size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp);

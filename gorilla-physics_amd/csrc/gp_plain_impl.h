@@ -60,16 +60,20 @@ struct PlainCtx {
   }
 };
 
-// Articulated-body forward dynamics of one lane; q, v, tau, vdot are the
-// lane's columns of [k][Bp] arrays. contacts = false leaves out the contact
-// and spring-contact forces (the velocity-stepping path). Returns false on a
-// non-positive D.
-__device__ inline bool plain_dynamics(const BucketArgs& a, int lane, Strided q, Strided v, Strided tau, Strided vdot,
-                                      bool contacts = true) {
+// Pass 1 of the dynamics for one lane, root to leaves: each body's joint
+// transform (PF_E = E, PF_R = r), velocity (PF_VEL), velocity-product
+// acceleration v x_m S qd (PF_CB), rigid-body inertia (PF_IA) and bias force
+// v x* (I v) - f_ext (PF_PA) with the contact and spring-contact forces, in
+// the aligned body frame. contacts = false leaves the forces out;
+// update_springs = false evaluates the spring contacts without changing their
+// registration (a query: a spring registering in this evaluation, or
+// detaching, exerts no force either way). Returns false where a spring
+// contact pulls into its half-space.
+__device__ inline bool plain_kinematics(const BucketArgs& a, int lane, Strided q, Strided v, bool contacts = true,
+                                        bool update_springs = true) {
   PlainCtx X{a, lane};
   const int nb = a.nb, Bp = a.Bp;
   bool ok = true;
-  // ---- pass 1: kinematics, velocities, bias forces, contact forces
   for (int i = 0; i < nb; ++i) {
     const int k = a.kind[i * Bp + lane];
     const int pr = a.parent[i];
@@ -144,6 +148,7 @@ __device__ inline bool plain_dynamics(const BucketArgs& a, int lane, Strided q, 
         auto sst = [&](int fld) -> double& { return a.ss[((size_t)fld * a.nsp + sc) * Bp + lane]; };
         auto hpar = [&](int h, int fld) { return a.hp[((size_t)fld * a.nh + h) * Bp + lane]; };
         if (sst(SS_REG) == 0.0) {
+          if (!update_springs) continue;  // registering exerts no force in this evaluation
           // the leg's tip: body origin + R direction l_rest; the first half-space holding it registers it
           const v3<double> tip = tw + spar(SF_L) * mul(Rw, v3<double>{spar(SF_DIR), spar(SF_DIR + 1), spar(SF_DIR + 2)});
           for (int h = 0; h < a.nh; ++h) {
@@ -166,7 +171,7 @@ __device__ inline bool plain_dynamics(const BucketArgs& a, int lane, Strided q, 
             const double force = -spar(SF_K) * (dist - spar(SF_L));  // energy_control.rs:13
             const v3<double> F = force * (-sdir);                    // at the body's origin: no torque about it
             f.v = f.v - mulT(Rw, F);
-          } else {
+          } else if (update_springs) {
             // detached: the leg keeps its last direction, a world vector stored as
             // the body-frame direction (contact.rs:176-181), here in the aligned frame
             sst(SS_REG) = 0.0;
@@ -181,6 +186,19 @@ __device__ inline bool plain_dynamics(const BucketArgs& a, int lane, Strided q, 
     X.stv(PF_PA, i, f);
     X.sti(i, I);
   }
+  return ok;
+}
+
+// Articulated-body forward dynamics of one lane; q, v, tau, vdot are the
+// lane's columns of [k][Bp] arrays. contacts = false leaves out the contact
+// and spring-contact forces (the velocity-stepping path). Returns false on a
+// non-positive D.
+__device__ inline bool plain_dynamics(const BucketArgs& a, int lane, Strided q, Strided v, Strided tau, Strided vdot,
+                                      bool contacts = true) {
+  PlainCtx X{a, lane};
+  const int nb = a.nb, Bp = a.Bp;
+  // ---- pass 1: kinematics, velocities, bias forces, contact forces
+  bool ok = plain_kinematics(a, lane, q, v, contacts);
   // ---- pass 2: articulated inertias and bias forces, leaves to root
   for (int i = nb - 1; i >= 0; --i) {
     const int k = a.kind[i * Bp + lane];

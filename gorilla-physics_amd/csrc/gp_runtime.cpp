@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <tuple>
@@ -419,9 +420,13 @@ int bucket_build(Bucket& b, const std::vector<const gp_model*>& lane_models, boo
 
 // Scratch of the runtime-topology kernels: room for the plain kernel (per-body
 // fields and Runge-Kutta stages) and for the velocity-stepping kernel.
+size_t bucket_scratch_doubles(const Bucket& b) {
+  return std::max<size_t>({(size_t)1, plain_scratch_doubles(b.nb, b.nq, b.nv, b.Bp),
+                           vs_scratch_doubles(b.nb, b.nq, b.nv, b.nc, b.nh, b.ns, b.Bp)});
+}
+
 int bucket_ensure_scratch(Bucket& b) {
-  const size_t need = std::max<size_t>({(size_t)1, plain_scratch_doubles(b.nb, b.nq, b.nv, b.Bp),
-                                        vs_scratch_doubles(b.nb, b.nq, b.nv, b.nc, b.nh, b.ns, b.Bp)});
+  const size_t need = bucket_scratch_doubles(b);
   if (b.d_scratch && b.scratch_doubles >= need) return GP_OK;
   if (b.d_scratch) {
     (void)hipFree(b.d_scratch);
@@ -464,12 +469,14 @@ static bool is_velocity_stepping(int integrator) {
   return integrator == GP_VELOCITY_STEPPING || integrator == GP_CCD_VELOCITY_STEPPING;
 }
 static bool bucket_is_fast(const gp_batch_opts& o, const Bucket& b, int integrator) {
-  return o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0 && !is_velocity_stepping(integrator);
+  // the one-body-per-lane tree kernel is semi-implicit Euler only
+  return o.kernel != GP_KERNEL_PLAIN && b.fast_topo >= 0 && !is_velocity_stepping(integrator) &&
+         !(b.fast_topo == TOPO_TREE && integrator != GP_SEMI_IMPLICIT_EULER);
 }
 static int bucket_lanes(const gp_batch_opts& o, const Bucket& b, int integrator) {
   // the tree-parallel quadruped kernel is semi-implicit Euler only
   const int tree = integrator == GP_SEMI_IMPLICIT_EULER ? o.lanes_per_tree : 1;
-  return fast_lanes(b.fast_topo, o.lanes_per_mech, tree);
+  return fast_lanes(b.fast_topo, o.lanes_per_mech, tree, b.nb);
 }
 static int group_block_size(const gp_batch_opts& o) { return o.block_size > 0 ? o.block_size : 128; }
 
@@ -511,9 +518,15 @@ static int ensure_group(gp_batch* bt, int integrator) {
   return GP_OK;
 }
 
-static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps, hipStream_t s, int64_t* launches) {
+// Every bucket's launches for n_steps steps. scratch_override: where the
+// runtime-topology and velocity-stepping kernels keep their per-lane working
+// set (nullptr: each bucket's own buffer, allocated on first use; else that
+// array, which must hold the largest bucket's, and no grouped launch, whose
+// table is allocated on first use).
+static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps, hipStream_t s, int64_t* launches,
+                            double* scratch_override = nullptr) {
   const gp_batch_opts& o = bt->opts;
-  const bool grouped = grouped_count(bt, integrator) > 0;
+  const bool grouped = scratch_override == nullptr && grouped_count(bt, integrator) > 0;
   if (grouped) {
     if (!bt->group_valid) return fail(GP_ERR_INVALID, "group table not prepared");
     int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
@@ -532,9 +545,10 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
     if (is_velocity_stepping(integrator)) {
       if (o.precision == GP_PREC_MIXED && o.kernel == GP_KERNEL_PLAIN)
         return fail(GP_ERR_UNSUPPORTED, "velocity stepping computes in FP64 only");
-      int st = bucket_ensure_scratch(b);
+      int st = scratch_override ? GP_OK : bucket_ensure_scratch(b);
       if (st) return st;
       BucketArgs a = bucket_args(b, 1, nullptr);
+      if (scratch_override) a.scratch = scratch_override;
       const int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
       for (int done = 0; done < n_steps; done += spl) {
         const int n = std::min(spl, n_steps - done);
@@ -559,9 +573,10 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
     } else {
       if (o.precision == GP_PREC_MIXED && o.kernel == GP_KERNEL_PLAIN)
         return fail(GP_ERR_UNSUPPORTED, "the plain kernel computes in FP64 only");
-      int st = bucket_ensure_scratch(b);
+      int st = scratch_override ? GP_OK : bucket_ensure_scratch(b);
       if (st) return st;
       BucketArgs a = bucket_args(b, 1, nullptr);
+      if (scratch_override) a.scratch = scratch_override;
       const int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
       for (int done = 0; done < n_steps; done += spl) {
         int e = launch_plain_step(a, dt, integrator, 0, std::min(spl, n_steps - done), s);
@@ -888,7 +903,7 @@ int gp_batch_set_tau(gp_batch* bt, const double* tau) {
   return transfer(bt, nullptr, tau, nullptr, nullptr, true, 1);
 }
 
-int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* stream) {
+static int check_step_args(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps) {
   if (!bt) return fail(GP_ERR_INVALID, "null batch");
   if (!std::isfinite(dt)) return fail(GP_ERR_INVALID, "dt is not finite");
   if (integrator < GP_SEMI_IMPLICIT_EULER || integrator > GP_CCD_VELOCITY_STEPPING)
@@ -898,47 +913,104 @@ int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* 
     for (const Bucket& b : bt->buckets)
       if (b.nsp > 0)  // simulate.rs:56-71 asserts the same
         return fail(GP_ERR_UNSUPPORTED, "Runge-Kutta on a state with spring contacts");
-  hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
-  int64_t launches = 0;
+  return GP_OK;
+}
+
+// Steps that one launch of every bucket covers; a rollout is a series of such
+// chunks and a shorter last one.
+static int chunk_steps(const gp_batch* bt) {
+  return bt->opts.steps_per_launch > 0 ? bt->opts.steps_per_launch : 100;
+}
+
+// The cached graph of one chunk of `steps` steps (one launch per bucket, or one
+// grouped launch), captured and instantiated on first use.
+static int chunk_graph(gp_batch* bt, double dt, int integrator, int steps, hipStream_t s, hipGraphExec_t* ex,
+                       int64_t* launches) {
+  const auto key = std::make_tuple(dt, integrator, steps);
+  auto it = bt->graphs.find(key);
+  if (it == bt->graphs.end()) {
+    // bucket scratch must exist before capture (no allocation inside a graph)
+    for (Bucket& b : bt->buckets) {
+      int st = bucket_ensure_scratch(b);
+      if (st) return st;
+    }
+    int64_t n = 0;
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int st = batch_launch_all(bt, dt, integrator, steps, s, &n);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (st) return st;
+    if (e != hipSuccess) return fail(GP_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
+    hipGraphExec_t x;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(GP_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e));
+    it = bt->graphs.emplace(key, x).first;
+    bt->graph_launches[key] = n;
+  }
+  *ex = it->second;
+  *launches = bt->graph_launches[key];
+  return GP_OK;
+}
+
+// Everything a rollout of n_steps needs before its first launch: the grouped
+// launch's table and, with graphs on, the chunk graphs.
+static int step_prepare(gp_batch* bt, double dt, int integrator, int n_steps, hipStream_t s) {
   if (grouped_count(bt, integrator) > 0) {
     int st = ensure_group(bt, integrator);
     if (st) return st;
   }
-  HIP_TRY(hipEventRecord(bt->ev0, s));
-  bool graph = bt->opts.use_graph != 0 && n_steps > 0;
-  if (graph) {
-    auto key = std::make_tuple(dt, integrator, n_steps);
-    auto it = bt->graphs.find(key);
-    if (it == bt->graphs.end()) {
-      // bucket scratch must exist before capture (no allocation inside a graph)
-      for (Bucket& b : bt->buckets) {
-        int st = bucket_ensure_scratch(b);
-        if (st) return st;
-      }
-      hipGraph_t g;
-      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int st = batch_launch_all(bt, dt, integrator, n_steps, s, &launches);
-      hipError_t e = hipStreamEndCapture(s, &g);
-      if (st) return st;
-      if (e != hipSuccess) return fail(GP_ERR_DEVICE, "graph capture: %s", hipGetErrorString(e));
-      hipGraphExec_t ex;
-      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return fail(GP_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(e));
-      it = bt->graphs.emplace(key, ex).first;
-      bt->graph_launches[key] = launches;
-    }
-    launches = bt->graph_launches[key];
-    HIP_TRY(hipGraphLaunch(it->second, s));
-  } else {
-    int st = batch_launch_all(bt, dt, integrator, n_steps, s, &launches);
+  if (!bt->opts.use_graph || n_steps <= 0) return GP_OK;
+  const int spl = chunk_steps(bt);
+  hipGraphExec_t ex;
+  int64_t l = 0;
+  if (n_steps >= spl) {
+    int st = chunk_graph(bt, dt, integrator, spl, s, &ex, &l);
     if (st) return st;
+  }
+  if (n_steps % spl) return chunk_graph(bt, dt, integrator, n_steps % spl, s, &ex, &l);
+  return GP_OK;
+}
+
+int gp_step_prepare(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* stream) {
+  int st = check_step_args(bt, dt, integrator, n_steps);
+  if (st) return st;
+  return step_prepare(bt, dt, integrator, n_steps, stream ? (hipStream_t)stream : bt->stream);
+}
+
+int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* stream) {
+  int st = check_step_args(bt, dt, integrator, n_steps);
+  if (st) return st;
+  hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
+  if ((st = step_prepare(bt, dt, integrator, n_steps, s))) return st;
+  int64_t launches = 0;
+  // the timing events bracket the launches only (any capture is done above)
+  HIP_TRY(hipEventRecord(bt->ev0, s));
+  if (bt->opts.use_graph && n_steps > 0) {
+    const int spl = chunk_steps(bt);
+    hipGraphExec_t ex;
+    int64_t l = 0;
+    if (n_steps >= spl) {
+      if ((st = chunk_graph(bt, dt, integrator, spl, s, &ex, &l))) return st;
+      for (int done = 0; done + spl <= n_steps; done += spl) {
+        HIP_TRY(hipGraphLaunch(ex, s));
+        launches += l;
+      }
+    }
+    if (n_steps % spl) {
+      if ((st = chunk_graph(bt, dt, integrator, n_steps % spl, s, &ex, &l))) return st;
+      HIP_TRY(hipGraphLaunch(ex, s));
+      launches += l;
+    }
+  } else {
+    if ((st = batch_launch_all(bt, dt, integrator, n_steps, s, &launches))) return st;
   }
   HIP_TRY(hipEventRecord(bt->ev1, s));
   bt->last_launches = launches;
   bt->timing_valid = true;
   return GP_OK;
 }
+
 
 int gp_batch_sync(gp_batch* bt) {
   if (!bt) return fail(GP_ERR_INVALID, "null batch");
@@ -963,6 +1035,52 @@ int gp_dynamics(gp_batch* bt, const double* tau, double* vdot) {
     if (e) return fail(GP_ERR_DEVICE, "dynamics launch: %s", hipGetErrorString((hipError_t)e));
   }
   return transfer(bt, nullptr, nullptr, nullptr, vdot, false, 2);
+}
+
+// M (instance-major, each nv_i x nv_i row-major) and / or c - tau_c (the v
+// flattening) of every instance at the batch's state
+static int joint_space_terms(gp_batch* bt, double* Mh, double* ch) {
+  std::vector<int64_t> moff(bt->n_inst + 1, 0);
+  for (int64_t i = 0; i < bt->n_inst; ++i) {
+    const int64_t nv = bt->voff[i + 1] - bt->voff[i];
+    moff[i + 1] = moff[i] + nv * nv;
+  }
+  HIP_TRY(hipStreamSynchronize(bt->stream));
+  for (Bucket& b : bt->buckets) {
+    const size_t rows = (size_t)b.nv * b.nv + b.nv;
+    if (b.nv == 0 || b.B == 0) continue;
+    int st = bucket_ensure_scratch(b);
+    if (st) return st;
+    double* d = nullptr;
+    hipError_t e = hipMalloc((void**)&d, rows * b.Bp * sizeof(double));
+    if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(terms): %s", hipGetErrorString(e));
+    std::vector<double> h(rows * b.Bp);
+    BucketArgs a = bucket_args(b, 1, nullptr);
+    int le = launch_terms(a, d, bt->stream);
+    if (!le) le = (int)hipStreamSynchronize(bt->stream);
+    if (!le) le = (int)hipMemcpy(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (le) return fail(GP_ERR_DEVICE, "joint-space terms: %s", hipGetErrorString((hipError_t)le));
+    const size_t Bp = b.Bp, nv = b.nv;
+    for (int l = 0; l < b.B; ++l) {
+      const int64_t i = b.inst[l];
+      if (Mh)
+        for (size_t k = 0; k < nv * nv; ++k) Mh[moff[i] + k] = h[k * Bp + l];
+      if (ch)
+        for (size_t k = 0; k < nv; ++k) ch[bt->voff[i] + k] = h[(nv * nv + k) * Bp + l];
+    }
+  }
+  return GP_OK;
+}
+
+int gp_mass_matrix(gp_batch* bt, double* M) {
+  if (!bt || !M) return fail(GP_ERR_INVALID, "null argument");
+  return joint_space_terms(bt, M, nullptr);
+}
+
+int gp_dynamics_bias(gp_batch* bt, double* c) {
+  if (!bt || !c) return fail(GP_ERR_INVALID, "null argument");
+  return joint_space_terms(bt, nullptr, c);
 }
 
 int gp_simulate(gp_batch* bt, double final_time, double dt, gp_control_fn control_fn, void* user, int32_t integrator,

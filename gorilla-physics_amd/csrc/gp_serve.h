@@ -44,7 +44,24 @@ struct GatherBatch {
   int* fault;        // [K]
 };
 
-int launch_admit(const PoolPtrs& p, const AdmitBatch& x, void* stream);
-int launch_gather(const PoolPtrs& p, const GatherBatch& g, void* stream);
+// Every pool's admissions (or finished slots) of one serving round, for one
+// launch: table entry e covers requests [start[e], start[e + 1]).
+struct AdmitTable {
+  int n;
+  int start[kMaxGroup + 1];
+  PoolPtrs p[kMaxGroup];
+  AdmitBatch x[kMaxGroup];
+};
+struct GatherTable {
+  int n;
+  int start[kMaxGroup + 1];
+  PoolPtrs p[kMaxGroup];
+  GatherBatch g[kMaxGroup];
+};
+
+// One workgroup per request; its lanes copy the request's values in parallel.
+// d_table and everything it points to are device memory.
+int launch_admit(const AdmitTable* d_table, int n_requests, void* stream);
+int launch_gather(const GatherTable* d_table, int n_requests, void* stream);
 
 }  // namespace gp

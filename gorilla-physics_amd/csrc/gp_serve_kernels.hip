@@ -1,9 +1,10 @@
 // gp_serve_kernels.hip — slot admission and retirement for the serving loop.
 //
-// admit: K requests, packed on the host one after another, are written into
-// their slots (lanes) of a pool's structure-of-arrays state and parameters in
-// one launch. gather: the final state of K finished slots is packed into a
-// compact buffer for one device-to-host copy.
+// admit: the requests of every pool admitted in one round, packed on the host
+// one after another, are written into their slots (lanes) of the pools'
+// structure-of-arrays state and parameters by one launch. gather: the final
+// state of every slot that finished in the round is packed into a compact
+// buffer for one device-to-host copy, by one launch.
 #include <hip/hip_runtime.h>
 
 #include "gp_internal.h"
@@ -11,61 +12,95 @@
 
 namespace gp {
 
-__global__ void __launch_bounds__(64) admit_kernel(PoolPtrs p, AdmitBatch x) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= x.K) return;
+// the table entry holding request r (uniform per workgroup)
+template <class T> __device__ __forceinline__ int entry_of(const T* t, int r) {
+  int e = 0;
+  while (e + 1 < t->n && r >= t->start[e + 1]) ++e;
+  return e;
+}
+
+// Workgroup r writes request r into its slot: kind, body / contact /
+// half-space parameters (both precisions), q, v, tau, a zero vdot and fault,
+// and the horizon. Lane j handles elements j, j + 64, ... of the request.
+__global__ void __launch_bounds__(64) admit_kernel(const AdmitTable* __restrict__ t) {
+  const int r = blockIdx.x;
+  const int e = entry_of(t, r);
+  const PoolPtrs& p = t->p[e];
+  const AdmitBatch& x = t->x[e];
+  const int k = r - t->start[e];
   const size_t Bp = p.Bp;
   const int l = x.lanes[k];
-  const double* bp = x.bp + (size_t)k * p.nb * NBF;
-  for (int i = 0; i < p.nb; ++i) {
-    p.kind[i * Bp + l] = x.kind[k * p.nb + i];
-    for (int f = 0; f < NBF; ++f) {
-      const double val = bp[i * NBF + f];
+  const int nbp = p.nb * NBF, ncp = p.nc * NCF, nhp = p.nh * NHF;
+  const int n_elem = p.nb + nbp + ncp + nhp + p.nq + 2 * p.nv;
+  for (int j = threadIdx.x; j < n_elem; j += blockDim.x) {
+    int m = j;
+    if (m < p.nb) { p.kind[m * Bp + l] = x.kind[k * p.nb + m]; continue; }
+    m -= p.nb;
+    if (m < nbp) {  // packed [nb][NBF] -> SoA [NBF][nb]
+      const int i = m / NBF, f = m % NBF;
+      const double val = x.bp[(size_t)k * nbp + m];
       p.bp[((size_t)f * p.nb + i) * Bp + l] = val;
       p.bpf[((size_t)f * p.nb + i) * Bp + l] = float(val);
+      continue;
     }
+    m -= nbp;
+    if (m < ncp) {
+      const int c = m / NCF, f = m % NCF;
+      const double val = x.cp[(size_t)k * ncp + m];
+      p.cp[((size_t)f * p.nc + c) * Bp + l] = val;
+      p.cpf[((size_t)f * p.nc + c) * Bp + l] = float(val);
+      continue;
+    }
+    m -= ncp;
+    if (m < nhp) {
+      const int h = m / NHF, f = m % NHF;
+      const double val = x.hp[(size_t)k * nhp + m];
+      p.hp[((size_t)f * p.nh + h) * Bp + l] = val;
+      p.hpf[((size_t)f * p.nh + h) * Bp + l] = float(val);
+      continue;
+    }
+    m -= nhp;
+    if (m < p.nq) { p.q[m * Bp + l] = x.q[(size_t)k * p.nq + m]; continue; }
+    m -= p.nq;
+    if (m < p.nv) {
+      p.v[m * Bp + l] = x.v[(size_t)k * p.nv + m];
+      p.vdot[m * Bp + l] = 0.0;
+      continue;
+    }
+    m -= p.nv;
+    p.tau[m * Bp + l] = x.tau[(size_t)k * p.nv + m];
   }
-  const double* cp = x.cp + (size_t)k * p.nc * NCF;
-  for (int c = 0; c < p.nc; ++c)
-    for (int f = 0; f < NCF; ++f) {
-      p.cp[((size_t)f * p.nc + c) * Bp + l] = cp[c * NCF + f];
-      p.cpf[((size_t)f * p.nc + c) * Bp + l] = float(cp[c * NCF + f]);
-    }
-  const double* hp = x.hp + (size_t)k * p.nh * NHF;
-  for (int h = 0; h < p.nh; ++h)
-    for (int f = 0; f < NHF; ++f) {
-      p.hp[((size_t)f * p.nh + h) * Bp + l] = hp[h * NHF + f];
-      p.hpf[((size_t)f * p.nh + h) * Bp + l] = float(hp[h * NHF + f]);
-    }
-  for (int j = 0; j < p.nq; ++j) p.q[j * Bp + l] = x.q[(size_t)k * p.nq + j];
-  for (int j = 0; j < p.nv; ++j) {
-    p.v[j * Bp + l] = x.v[(size_t)k * p.nv + j];
-    p.tau[j * Bp + l] = x.tau[(size_t)k * p.nv + j];
-    p.vdot[j * Bp + l] = 0.0;
+  if (threadIdx.x == 0) {
+    p.fault[l] = 0;
+    p.active[l] = x.steps[k];
   }
-  p.fault[l] = 0;
-  p.active[l] = x.steps[k];
 }
 
-__global__ void __launch_bounds__(64) gather_kernel(PoolPtrs p, GatherBatch g) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.K) return;
+// Workgroup r packs finished slot r's q, v and fault.
+__global__ void __launch_bounds__(64) gather_kernel(const GatherTable* __restrict__ t) {
+  const int r = blockIdx.x;
+  const int e = entry_of(t, r);
+  const PoolPtrs& p = t->p[e];
+  const GatherBatch& g = t->g[e];
+  const int k = r - t->start[e];
   const size_t Bp = p.Bp;
   const int l = g.lanes[k];
-  for (int j = 0; j < p.nq; ++j) g.q[(size_t)k * p.nq + j] = p.q[j * Bp + l];
-  for (int j = 0; j < p.nv; ++j) g.v[(size_t)k * p.nv + j] = p.v[j * Bp + l];
-  g.fault[k] = p.fault[l];
+  for (int j = threadIdx.x; j < p.nq + p.nv; j += blockDim.x) {
+    if (j < p.nq) g.q[(size_t)k * p.nq + j] = p.q[j * Bp + l];
+    else g.v[(size_t)k * p.nv + (j - p.nq)] = p.v[(j - p.nq) * Bp + l];
+  }
+  if (threadIdx.x == 0) g.fault[k] = p.fault[l];
 }
 
-int launch_admit(const PoolPtrs& p, const AdmitBatch& x, void* stream) {
-  if (x.K <= 0) return 0;
-  hipLaunchKernelGGL(admit_kernel, dim3((x.K + 63) / 64), dim3(64), 0, (hipStream_t)stream, p, x);
+int launch_admit(const AdmitTable* d_table, int n_requests, void* stream) {
+  if (n_requests <= 0) return 0;
+  hipLaunchKernelGGL(admit_kernel, dim3(n_requests), dim3(64), 0, (hipStream_t)stream, d_table);
   return (int)hipGetLastError();
 }
 
-int launch_gather(const PoolPtrs& p, const GatherBatch& g, void* stream) {
-  if (g.K <= 0) return 0;
-  hipLaunchKernelGGL(gather_kernel, dim3((g.K + 63) / 64), dim3(64), 0, (hipStream_t)stream, p, g);
+int launch_gather(const GatherTable* d_table, int n_requests, void* stream) {
+  if (n_requests <= 0) return 0;
+  hipLaunchKernelGGL(gather_kernel, dim3(n_requests), dim3(64), 0, (hipStream_t)stream, d_table);
   return (int)hipGetLastError();
 }
 

@@ -79,6 +79,8 @@ double step_cost(const Bucket& b, int precision, int lanes_per_tree) {
   const bool mixed = precision == GP_PREC_MIXED;
   const int t = b.fast_topo;
   if (t >= 1 && t <= 12) return (mixed ? chain_mixed : chain_f64)[t - 1] * (b.all_rev ? 1.0 : 1.2);
+  if (t > TOPO_WELDED && t <= TOPO_WELDED + kMaxWeldedJoints)  // a welded base costs nothing per step
+    return (mixed ? chain_mixed : chain_f64)[t - TOPO_WELDED - 1] * (b.all_rev ? 1.0 : 1.2);
   if (t == TOPO_QUAD && lanes_per_tree == 1) return mixed ? 29.4 : 116.0;
   if (t == TOPO_QUAD) return mixed ? 7.0 : 14.3;   // one lane per leg
   if (t == TOPO_FREE) return mixed ? 1.4 : 1.7;    // four lanes per body, 8 contact points
@@ -102,14 +104,15 @@ struct gp_server {
   int64_t next_id = 1;
   std::chrono::steady_clock::time_point t0;
   hipStream_t stream = nullptr;
-  GroupTable* d_group = nullptr;
+  GroupTable* d_group = nullptr;  // n_tables tables of up to kMaxGroup pools each
+  int d_group_cap = 0;
   bool group_dirty = true;
-  int group_blocks = 0;
+  std::vector<int> group_blocks;  // per table
   char* d_stage = nullptr;
   size_t d_stage_cap = 0;
   char* h_stage = nullptr;
   size_t h_stage_cap = 0;
-  double rounds = 0, env_steps = 0, busy_s = 0, admitted = 0, completed = 0;
+  double rounds = 0, env_steps = 0, busy_s = 0, admitted = 0, completed = 0, admit_s = 0, retire_s = 0;
   size_t mem_used = 0, budget = 0;
   int64_t running = 0;
 
@@ -209,42 +212,57 @@ int find_pool(gp_server* s, const Request& r, int* out) {
   return GP_OK;
 }
 
+// The grouped launch tables: every live pool in one table of up to kMaxGroup
+// pools, and as many tables (one launch each) as the pools need.
 int rebuild_group(gp_server* s) {
-  GroupTable t{};
+  std::vector<GroupTable> tabs;
+  std::vector<int> blocks;
   const int block = s->o.block_size;
-  int blocks = 0;
   for (auto& pp : s->pools) {
     Pool& p = *pp;
     if (p.bytes == 0) continue;  // released
-    if (t.n >= kMaxGroup) return fail(GP_ERR_UNSUPPORTED, "more than %d live pools", kMaxGroup);
-    const int lanes = fast_lanes(p.b.fast_topo, s->o.lanes_per_mech, s->o.lanes_per_tree);
+    if (tabs.empty() || tabs.back().n == kMaxGroup) {
+      tabs.emplace_back();
+      std::memset(&tabs.back(), 0, sizeof(GroupTable));
+      blocks.push_back(0);
+    }
+    GroupTable& t = tabs.back();
+    const int lanes = fast_lanes(p.b.fast_topo, s->o.lanes_per_mech, s->o.lanes_per_tree, p.b.nb);
     BucketArgs a = bucket_args(p.b, lanes, nullptr);
     a.dt = p.dt;
     a.n_steps = p.chunk;
     t.args[t.n] = a;
     t.topo[t.n] = p.b.fast_topo;
-    t.block_start[t.n] = blocks;
-    blocks += (int)(((int64_t)p.b.B * lanes + block - 1) / block);
+    t.block_start[t.n] = blocks.back();
+    blocks.back() += (int)(((int64_t)p.b.B * lanes + block - 1) / block);
     ++t.n;
+    t.block_start[t.n] = blocks.back();
   }
-  t.block_start[t.n] = blocks;
-  if (!s->d_group) {
-    hipError_t e = hipMalloc((void**)&s->d_group, sizeof(GroupTable));
+  if ((int)tabs.size() > s->d_group_cap) {
+    if (s->d_group) (void)hipFree(s->d_group);
+    s->d_group = nullptr;
+    s->d_group_cap = 0;
+    hipError_t e = hipMalloc((void**)&s->d_group, sizeof(GroupTable) * tabs.size());
     if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(group): %s", hipGetErrorString(e));
+    s->d_group_cap = (int)tabs.size();
   }
-  hipError_t e = hipMemcpy(s->d_group, &t, sizeof t, hipMemcpyHostToDevice);
-  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "hipMemcpy(group): %s", hipGetErrorString(e));
+  if (!tabs.empty()) {
+    hipError_t e = hipMemcpy(s->d_group, tabs.data(), sizeof(GroupTable) * tabs.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(GP_ERR_DEVICE, "hipMemcpy(group): %s", hipGetErrorString(e));
+  }
   s->group_blocks = blocks;
   s->group_dirty = false;
   return GP_OK;
 }
 
-// Pack the admissions of one round and write them into their pools.
-int admit_round(gp_server* s, std::map<int, std::vector<int64_t>>& adm) {
-  // layout per pool: lanes, kind, steps (ints) then bp, cp, hp, q, v, tau (doubles)
+// Pack the admissions of one round (every pool's) behind a table of them,
+// copy the whole in one transfer and write them into their slots with one
+// launch.
+int admit_pools(gp_server* s, std::map<int, std::vector<int64_t>>& adm) {
+  // layout: the table, then per pool: lanes, kind, steps (ints) then bp, cp, hp, q, v, tau (doubles)
   struct Off { size_t lanes, kind, steps, bp, cp, hp, q, v, tau; int K; };
   std::map<int, Off> offs;
-  size_t at = 0;
+  size_t at = round_up<double>(sizeof(AdmitTable));
   for (auto& kv : adm) {
     const Bucket& b = s->pools[kv.first]->b;
     const size_t K = kv.second.size();
@@ -264,6 +282,10 @@ int admit_round(gp_server* s, std::map<int, std::vector<int64_t>>& adm) {
   int st = ensure_stage(s, at);
   if (st) return st;
   char* h = s->h_stage;
+  char* d = s->d_stage;
+  AdmitTable* tab = (AdmitTable*)h;
+  std::memset(tab, 0, sizeof *tab);
+  int total = 0;
   for (auto& kv : adm) {
     const Off& o = offs[kv.first];
     const Bucket& b = s->pools[kv.first]->b;
@@ -280,28 +302,32 @@ int admit_round(gp_server* s, std::map<int, std::vector<int64_t>>& adm) {
       std::memcpy(h + o.v + (size_t)k * b.nv * 8, r.v0.data(), (size_t)b.nv * 8);
       std::memcpy(h + o.tau + (size_t)k * b.nv * 8, r.tau.data(), (size_t)b.nv * 8);
     }
+    tab->start[tab->n] = total;
+    tab->p[tab->n] = pool_ptrs(*s->pools[kv.first]);
+    tab->x[tab->n] = AdmitBatch{o.K, (const int*)(d + o.lanes), (const int*)(d + o.kind), (const int*)(d + o.steps),
+                                (const double*)(d + o.bp), (const double*)(d + o.cp), (const double*)(d + o.hp),
+                                (const double*)(d + o.q), (const double*)(d + o.v), (const double*)(d + o.tau)};
+    total += o.K;
+    ++tab->n;
   }
-  hipError_t e = hipMemcpyAsync(s->d_stage, s->h_stage, at, hipMemcpyHostToDevice, s->stream);
+  tab->start[tab->n] = total;
+  hipError_t e = hipMemcpyAsync(d, h, at, hipMemcpyHostToDevice, s->stream);
   if (e != hipSuccess) return fail(GP_ERR_DEVICE, "admission copy: %s", hipGetErrorString(e));
-  for (auto& kv : adm) {
-    const Off& o = offs[kv.first];
-    char* d = s->d_stage;
-    AdmitBatch x{o.K, (const int*)(d + o.lanes), (const int*)(d + o.kind), (const int*)(d + o.steps),
-                 (const double*)(d + o.bp), (const double*)(d + o.cp), (const double*)(d + o.hp),
-                 (const double*)(d + o.q), (const double*)(d + o.v), (const double*)(d + o.tau)};
-    int err = launch_admit(pool_ptrs(*s->pools[kv.first]), x, s->stream);
-    if (err) return fail(GP_ERR_DEVICE, "admit launch: %s", hipGetErrorString((hipError_t)err));
-  }
+  int err = launch_admit((const AdmitTable*)d, total, s->stream);
+  if (err) return fail(GP_ERR_DEVICE, "admit launch: %s", hipGetErrorString((hipError_t)err));
   // the pinned staging is reused next round: wait for the copy
   e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) return fail(GP_ERR_DEVICE, "admission sync: %s", hipGetErrorString(e));
   return GP_OK;
 }
 
-int retire_round(gp_server* s, std::map<int, std::vector<int64_t>>& fin) {
+// Pack every slot that finished this round (one launch behind a table), copy
+// the results back in one transfer and complete their requests.
+int retire_pools(gp_server* s, std::map<int, std::vector<int64_t>>& fin) {
   struct Off { size_t lanes, q, v, fault; int K; };
   std::map<int, Off> offs;
-  size_t at = 0;
+  const size_t table_bytes = round_up<double>(sizeof(GatherTable));
+  size_t at = table_bytes;
   for (auto& kv : fin) {
     const Bucket& b = s->pools[kv.first]->b;
     const size_t K = kv.second.size();
@@ -316,20 +342,26 @@ int retire_round(gp_server* s, std::map<int, std::vector<int64_t>>& fin) {
   int st = ensure_stage(s, at);
   if (st) return st;
   char* h = s->h_stage;
+  char* d = s->d_stage;
+  GatherTable* tab = (GatherTable*)h;
+  std::memset(tab, 0, sizeof *tab);
+  int total = 0;
   for (auto& kv : fin) {
     const Off& o = offs[kv.first];
     for (int k = 0; k < o.K; ++k) ((int*)(h + o.lanes))[k] = s->req[kv.second[k]].slot;
+    tab->start[tab->n] = total;
+    tab->p[tab->n] = pool_ptrs(*s->pools[kv.first]);
+    tab->g[tab->n] = GatherBatch{o.K, (const int*)(d + o.lanes), (double*)(d + o.q), (double*)(d + o.v),
+                                 (int*)(d + o.fault)};
+    total += o.K;
+    ++tab->n;
   }
-  hipError_t e = hipMemcpyAsync(s->d_stage, s->h_stage, at, hipMemcpyHostToDevice, s->stream);
+  tab->start[tab->n] = total;
+  hipError_t e = hipMemcpyAsync(d, h, at, hipMemcpyHostToDevice, s->stream);
   if (e != hipSuccess) return fail(GP_ERR_DEVICE, "retire copy: %s", hipGetErrorString(e));
-  for (auto& kv : fin) {
-    const Off& o = offs[kv.first];
-    char* d = s->d_stage;
-    GatherBatch g{o.K, (const int*)(d + o.lanes), (double*)(d + o.q), (double*)(d + o.v), (int*)(d + o.fault)};
-    int err = launch_gather(pool_ptrs(*s->pools[kv.first]), g, s->stream);
-    if (err) return fail(GP_ERR_DEVICE, "gather launch: %s", hipGetErrorString((hipError_t)err));
-  }
-  e = hipMemcpyAsync(s->h_stage, s->d_stage, at, hipMemcpyDeviceToHost, s->stream);
+  int err = launch_gather((const GatherTable*)d, total, s->stream);
+  if (err) return fail(GP_ERR_DEVICE, "gather launch: %s", hipGetErrorString((hipError_t)err));
+  e = hipMemcpyAsync(h + table_bytes, d + table_bytes, at - table_bytes, hipMemcpyDeviceToHost, s->stream);
   if (e != hipSuccess) return fail(GP_ERR_DEVICE, "result copy: %s", hipGetErrorString(e));
   e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) return fail(GP_ERR_DEVICE, "round sync: %s", hipGetErrorString(e));
@@ -354,6 +386,29 @@ int retire_round(gp_server* s, std::map<int, std::vector<int64_t>>& fin) {
     }
   }
   return GP_OK;
+}
+
+// A round's admissions or retirements, kMaxGroup pools (one table, one
+// launch) at a time.
+template <class F>
+int by_tables(std::map<int, std::vector<int64_t>>& all, F&& f) {
+  std::map<int, std::vector<int64_t>> part;
+  for (auto& kv : all) {
+    part.emplace(kv.first, std::move(kv.second));
+    if (part.size() == (size_t)kMaxGroup) {
+      if (int st = f(part)) return st;
+      part.clear();
+    }
+  }
+  return part.empty() ? GP_OK : f(part);
+}
+
+int admit_round(gp_server* s, std::map<int, std::vector<int64_t>>& adm) {
+  return by_tables(adm, [s](std::map<int, std::vector<int64_t>>& m) { return admit_pools(s, m); });
+}
+
+int retire_round(gp_server* s, std::map<int, std::vector<int64_t>>& fin) {
+  return by_tables(fin, [s](std::map<int, std::vector<int64_t>>& m) { return retire_pools(s, m); });
 }
 
 int run_rounds(gp_server* s, int max_rounds, int* rounds_done) {
@@ -388,8 +443,10 @@ int run_rounds(gp_server* s, int max_rounds, int* rounds_done) {
     }
     s->ready.swap(still);
     if (!adm.empty()) {
+      const double t_adm = s->now();
       int st = admit_round(s, adm);
       if (st) return st;
+      s->admit_s += s->now() - t_adm;
     }
     if (s->running == 0) {
       if (s->ready.empty() && s->future.empty()) break;
@@ -411,8 +468,15 @@ int run_rounds(gp_server* s, int max_rounds, int* rounds_done) {
       if (st) return st;
     }
     const double ta = s->now();
-    int e = launch_grouped_step(s->d_group, s->group_blocks, s->o.precision, 0.0, C, s->o.block_size, s->stream);
-    if (e) return fail(GP_ERR_DEVICE, "grouped launch: %s", hipGetErrorString((hipError_t)e));
+    for (size_t t = 0; t < s->group_blocks.size(); ++t) {
+      int e = launch_grouped_step(s->d_group + t, s->group_blocks[t], s->o.precision, 0.0, C, s->o.block_size,
+                                  s->stream);
+      if (e) return fail(GP_ERR_DEVICE, "grouped launch: %s", hipGetErrorString((hipError_t)e));
+    }
+    // the round's steps end before its retirements are packed (the stream
+    // orders them anyway; the wait keeps the step out of retire_s)
+    hipError_t he = hipStreamSynchronize(s->stream);
+    if (he != hipSuccess) return fail(GP_ERR_DEVICE, "round sync: %s", hipGetErrorString(he));
     // host mirror of the kernel's count-down
     std::map<int, std::vector<int64_t>> fin;
     for (size_t pi = 0; pi < s->pools.size(); ++pi) {
@@ -429,11 +493,10 @@ int run_rounds(gp_server* s, int max_rounds, int* rounds_done) {
       }
     }
     if (!fin.empty()) {
+      const double t_ret = s->now();
       int st = retire_round(s, fin);
       if (st) return st;
-    } else {
-      hipError_t he = hipStreamSynchronize(s->stream);
-      if (he != hipSuccess) return fail(GP_ERR_DEVICE, "round sync: %s", hipGetErrorString(he));
+      s->retire_s += s->now() - t_ret;
     }
     s->busy_s += s->now() - ta;
     s->rounds += 1;
@@ -577,6 +640,8 @@ int gp_server_stats(const gp_server* s, double* st) {
   st[5] = (double)std::count_if(s->pools.begin(), s->pools.end(), [](const std::unique_ptr<Pool>& p) { return p->bytes > 0; });
   st[6] = (double)s->mem_used;
   st[7] = (double)s->budget;
+  st[8] = s->admit_s;
+  st[9] = s->retire_s;
   return GP_OK;
 }
 

@@ -12,7 +12,8 @@ def _dp(a):
 
 
 class Server:
-    STATS = ("rounds", "env_steps", "busy_s", "admitted", "completed", "pools", "pool_bytes", "budget_bytes")
+    STATS = ("rounds", "env_steps", "busy_s", "admitted", "completed", "pools", "pool_bytes", "budget_bytes",
+             "admit_s", "retire_s")
 
     def __init__(self, precision=abi.GP_PREC_MIXED, chunk_steps=100, slots_per_pool=4096, block_size=0,
                  mem_budget_bytes=0, lanes_per_mech=0, lanes_per_tree=0):
@@ -78,6 +79,6 @@ class Server:
         return q[:m.nq], v[:m.nv], lat.value, bool(fault.value)
 
     def stats(self):
-        s = np.zeros(8)
+        s = np.zeros(len(self.STATS))
         abi.check(self.lib.gp_server_stats(self.handle, _dp(s)), self.lib)
         return dict(zip(self.STATS, s.tolist()))

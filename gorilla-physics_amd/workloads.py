@@ -21,7 +21,7 @@ from . import abi
 from .mechanism import (GRAVITY, Isometry3, ModelDesc, SpatialInertia, UnitQuaternion, body_desc, contact_desc,
                         halfspace_desc, HalfSpace)
 
-WORKLOAD_IDS = {"acrobot": 1, "arms": 2, "mixed": 3, "cubes": 4, "thrown_cubes": 5, "so101": 6}
+WORKLOAD_IDS = {"acrobot": 1, "arms": 2, "mixed": 3, "cubes": 4, "thrown_cubes": 5, "so101": 6, "trees": 7}
 
 
 def stream(seed, workload, field):
@@ -163,9 +163,8 @@ def cube_desc(k_point=12500.0, alpha=1.0 / 150.0, mu=0.5, m=1.0, l=1.0):
 def so101(n=65536, seed=0):
     """SO-101 arms (builders/mod.rs:252: welded base, six revolutes, the
     URDF's inertias): joint angles U(-pi, pi), velocities N(0, 0.5), held
-    torques N(0, 0.02) N m, RK4 at dt 1e-3 for 1000 steps. The arm has no
-    compiled topology (its root joint is fixed), so it runs on the
-    runtime-topology FP64 kernel."""
+    torques N(0, 0.02) N m, RK4 at dt 1e-3 for 1000 steps. Its welded base
+    and six-joint chain run on the welded-base compiled kernels."""
     from . import builders
     d = builders.build_so101().model_desc()
     q = stream(seed, "so101", 1).uniform(-np.pi, np.pi, 6 * n)
@@ -222,3 +221,160 @@ def cubes(n=16384, seed=0):
     return dict(models=[d], counts=[n], q=q.reshape(-1), v=v.reshape(-1), tau=np.zeros(6 * n),
                 dt=1e-4, steps=5000, name="cubes")
 
+
+
+# ----------------------------------------------------------------- trees of 16+ bodies
+def lowest_contact_z(d, q):
+    """World height of the lowest contact point of a model description at
+    configuration q (forward kinematics of the joint chain, mechanism.rs:153-170)."""
+    world, qo = [], 0
+    for i in range(d.n_bodies):
+        b = d._bodies[i]
+        iso = Isometry3(UnitQuaternion(*b.x_tree_quat), np.array(b.x_tree_trans))
+        axis = np.array(b.axis)
+        if b.jtype == abi.GP_JOINT_REVOLUTE:
+            iso = iso * Isometry3(UnitQuaternion.from_axis_angle(axis, q[qo]))
+            qo += 1
+        elif b.jtype == abi.GP_JOINT_PRISMATIC:
+            iso = iso * Isometry3(None, axis * q[qo])
+            qo += 1
+        elif b.jtype == abi.GP_JOINT_FLOATING:
+            iso = iso * Isometry3(UnitQuaternion(*q[qo:qo + 4]), np.array(q[qo + 4:qo + 7]))
+            qo += 7
+        world.append(iso if b.parent < 0 else world[b.parent] * iso)
+    zs = [(world[c.body].rotation.to_matrix() @ np.array(c.p) + world[c.body].translation)[2]
+          for c in (d._contacts[k] for k in range(d.desc.n_contacts))]
+    return min(zs) if zs else 0.0
+
+
+def hexapod_desc(p, ground=True):
+    """Cuboid base on a floating joint and six legs of [hip-z, hip-y, knee-y]
+    revolute joints (19 bodies, nq = 25, nv = 24), one contact point at each
+    shank's tip, over the half-space z = 0."""
+    bw, bd, bh, bm = p["bw"], p["bd"], p["bh"], p["bm"]
+    bodies = [body_desc(-1, abi.GP_JOINT_FLOATING, [0, 0, 0], Isometry3.identity(),
+                        SpatialInertia.cuboid_at([0, 0, 0], bm, bw, bd, bh, "base"))]
+    contacts = []
+    for leg in range(6):
+        sx, sy = (-1, 0, 1)[leg % 3], (-1, 1)[leg // 3]
+        lt, ls = p["lt"][leg], p["ls"][leg]
+        hz = len(bodies)
+        bodies.append(body_desc(0, abi.GP_JOINT_REVOLUTE, [0, 0, 1],
+                                Isometry3.translation_xyz(sx * bw / 2, sy * bd / 2, 0.0),
+                                SpatialInertia.cuboid_at([0, 0, 0], 0.1, 0.04, 0.04, 0.04, "hz")))
+        hy = len(bodies)
+        bodies.append(body_desc(hz, abi.GP_JOINT_REVOLUTE, [0, 1, 0], Isometry3.identity(),
+                                SpatialInertia.cuboid_at([0, sy * lt / 2, 0], p["mt"][leg], 0.03, lt, 0.03, "th")))
+        kn = len(bodies)
+        bodies.append(body_desc(hy, abi.GP_JOINT_REVOLUTE, [0, 1, 0], Isometry3.translation_xyz(0.0, sy * lt, 0.0),
+                                SpatialInertia.cuboid_at([0, 0, -ls / 2], p["ms"][leg], 0.03, 0.03, ls, "sh")))
+        contacts.append(contact_desc(kn, [0.0, 0.0, -ls], 20e3))
+    hs = [halfspace_desc(HalfSpace.new_with_params([0.0, 0.0, 1.0], 0.0, 0.9, 0.5))] if ground else []
+    return ModelDesc(bodies, contacts if ground else [], hs)
+
+
+def humanoid_desc(p):
+    """The reference's biped (builders/biped_builder.rs:12: floating base, two
+    legs of pelvis, hip, thigh, calf, ankle and foot, the feet's eight
+    corners as contact points) with two arms of [shoulder-x, shoulder-y,
+    elbow-y] revolute joints on the base: 19 bodies, over z = 0."""
+    from . import builders
+    st = builders.build_biped()
+    st.add_halfspace(HalfSpace.new_with_params([0.0, 0.0, 1.0], 0.0, 0.9, 0.5))
+    d = st.model_desc()
+    bodies = [d._bodies[i] for i in range(d.n_bodies)]
+    for side in (-1.0, 1.0):
+        sx = len(bodies)
+        bodies.append(body_desc(0, abi.GP_JOINT_REVOLUTE, [1, 0, 0],
+                                Isometry3.translation_xyz(side * 0.06, 0.0, 0.08),
+                                SpatialInertia.cuboid_at([0, 0, 0], 0.05, 0.03, 0.03, 0.03, "sx")))
+        sy = len(bodies)
+        bodies.append(body_desc(sx, abi.GP_JOINT_REVOLUTE, [0, 1, 0], Isometry3.identity(),
+                                SpatialInertia.cuboid_at([0, 0, -p["la"] / 2], p["ma"], 0.03, 0.03, p["la"], "ua")))
+        bodies.append(body_desc(sy, abi.GP_JOINT_REVOLUTE, [0, 1, 0], Isometry3.translation_xyz(0.0, 0.0, -p["la"]),
+                                SpatialInertia.cuboid_at([0, 0, -p["lf"] / 2], p["mf"], 0.03, 0.03, p["lf"], "fa")))
+    contacts = [d._contacts[i] for i in range(d.desc.n_contacts)]
+    hs = [d._halfspaces[i] for i in range(d.desc.n_halfspaces)]
+    return ModelDesc(bodies, contacts, hs)
+
+
+def random_tree_shape(rng, nb):
+    """Topology of a random tree of nb bodies: each body after the first hung
+    from one of the bodies before it; joints revolute (70 %), prismatic
+    (15 %) or fixed (15 %); a contact point on every fourth body."""
+    parents = [-1] + [int(rng.integers(0, i)) for i in range(1, nb)]
+    u = rng.random(nb)
+    jtypes = [abi.GP_JOINT_FLOATING] + [abi.GP_JOINT_REVOLUTE if u[i] < 0.7 else
+                                        (abi.GP_JOINT_PRISMATIC if u[i] < 0.85 else abi.GP_JOINT_FIXED)
+                                        for i in range(1, nb)]
+    return parents, jtypes
+
+
+def random_tree_desc(rng, nb, shape=None):
+    """A random tree of nb bodies (random_tree_shape, or the given (parents,
+    joint types)) on a floating root: random coordinate axes, random boxes
+    with offset centres of mass, prismatic joints with springs, a contact
+    point at the far end of every fourth body, over z = 0."""
+    from .mechanism import JointSpring
+    parents, jtypes = shape if shape is not None else random_tree_shape(rng, nb)
+    bodies = [body_desc(-1, abi.GP_JOINT_FLOATING, [0, 0, 0], Isometry3.identity(),
+                        SpatialInertia.cuboid_at([0, 0, 0], rng.uniform(2, 5), 0.4, 0.3, 0.2, "root"))]
+    contacts = []
+    for i in range(1, nb):
+        jt = jtypes[i]
+        l = rng.uniform(0.05, 0.15)
+        iso = Isometry3.translation_xyz(*rng.uniform(-0.15, 0.15, 2), -l)
+        inertia = SpatialInertia.cuboid_at([rng.uniform(-0.02, 0.02), rng.uniform(-0.02, 0.02), -l / 2],
+                                           rng.uniform(0.1, 1.0), 0.03, 0.03, l, f"b{i}")
+        spring = JointSpring(rng.uniform(50, 500), rng.uniform(-0.05, 0.05)) if jt == abi.GP_JOINT_PRISMATIC else None
+        bodies.append(body_desc(parents[i], jt, _axis(rng.integers(0, 3)), iso, inertia, spring))
+        if i % 4 == 0:
+            contacts.append(contact_desc(i, [0.0, 0.0, -l], 20e3))
+    hs = [halfspace_desc(HalfSpace.new_with_params([0.0, 0.0, 1.0], 0.0, 0.9, 0.5))]
+    return ModelDesc(bodies, contacts, hs)
+
+
+def trees(n=16384, seed=0, random_fraction=0.25):
+    """Trees of 16 or more bodies on a ground plane: hexapods (19 bodies, six
+    feet), humanoids (the biped plus two arms, 19 bodies, 16 foot corners),
+    half each of the non-random share, and random trees of 16-40 bodies
+    (eight random topologies, each instance with its own masses, lengths and
+    axes); joint angles N(0, 0.3) (prismatic N(0, 0.02)), the base raised
+    so that the lowest contact point starts between 3 mm inside the ground and
+    1 cm above it, velocities N(0, 0.3), held torques N(0, 0.02); dt 1e-4
+    (at 1e-3 the reference's explicit contact law, stiff normal force and
+    regularised Coulomb friction on 0.1 kg feet, diverges in the oracle itself
+    on most trees within a thousand steps)."""
+    kind = stream(seed, "trees", 1).random(n)
+    lt = stream(seed, "trees", 2).uniform(0.1, 0.25, (n, 6))
+    ls = stream(seed, "trees", 3).uniform(0.15, 0.3, (n, 6))
+    mt = stream(seed, "trees", 4).uniform(0.1, 0.4, (n, 6))
+    ms = stream(seed, "trees", 5).uniform(0.1, 0.4, (n, 6))
+    base = stream(seed, "trees", 6).uniform(0.3, 0.6, n)
+    arms = stream(seed, "trees", 7).uniform(0.1, 0.3, (n, 4))
+    rng = np.random.Generator(np.random.Philox(key=[int(seed), WORKLOAD_IDS["trees"] * 1000 + 50]))
+    srng = np.random.Generator(np.random.Philox(key=[int(seed), WORKLOAD_IDS["trees"] * 1000 + 51]))
+    shapes = [random_tree_shape(srng, int(srng.integers(16, 41))) for _ in range(8)]
+    pick = stream(seed, "trees", 10).integers(0, len(shapes), n)
+    rq = stream(seed, "trees", 8)
+    rv = stream(seed, "trees", 9)
+    models, qs, vs, taus = [], [], [], []
+    for i in range(n):
+        if kind[i] < random_fraction:
+            d = random_tree_desc(rng, len(shapes[pick[i]][0]), shapes[pick[i]])
+        elif kind[i] < random_fraction + (1 - random_fraction) / 2:
+            d = hexapod_desc(dict(bw=0.5, bd=0.3, bh=0.1, bm=3.0, lt=lt[i], ls=ls[i], mt=mt[i], ms=ms[i]))
+        else:
+            d = humanoid_desc(dict(la=arms[i, 0], lf=arms[i, 1], ma=arms[i, 2], mf=arms[i, 3]))
+        q = np.zeros(d.nq)
+        q[3] = 1.0
+        q[7:] = rq.normal(0.0, 0.3, d.nq - 7)
+        q[7:] *= [0.02 / 0.3 if d._bodies[b].jtype == abi.GP_JOINT_PRISMATIC else 1.0
+                  for b in range(1, d.n_bodies) if d._bodies[b].jtype != abi.GP_JOINT_FIXED]
+        q[6] = -lowest_contact_z(d, q) + (base[i] - 0.3) / 0.3 * 0.013 - 0.003
+        models.append(d)
+        qs.append(q)
+        vs.append(rv.normal(0.0, 0.3, d.nv))
+        taus.append(np.concatenate([np.zeros(6), rv.normal(0.0, 0.02, d.nv - 6)]))
+    return dict(models=models, counts=[1] * n, q=np.concatenate(qs), v=np.concatenate(vs),
+                tau=np.concatenate(taus), dt=1e-4, steps=1000, name="trees")

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GP_ABI_VERSION 4
+#define GP_ABI_VERSION 5
 
 /* ---- status codes (replace the reference's panic!/expect sites) ---------- */
 enum {
@@ -249,6 +249,12 @@ int gp_batch_get_vdot_dev(gp_batch* batch, double* vdot, void* stream);
  * the held torques. Asynchronous on `stream` (a hipStream_t; NULL = the batch's
  * own stream). */
 int gp_step(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, void* stream);
+/* Do now the one-time work that gp_step(batch, dt, integrator, n_steps) would
+ * do on its first call: the grouped launch's table, bucket scratch and, with
+ * use_graph, the hipGraphs it replays (one per chunk of steps_per_launch steps
+ * and one for a shorter last chunk), so that later gp_step calls only launch.
+ * Nothing is stepped. No reference counterpart (setup of the batched launch). */
+int gp_step_prepare(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, void* stream);
 /* Block until the batch's work on its stream is done. */
 int gp_batch_sync(gp_batch* batch);
 
@@ -260,6 +266,25 @@ int gp_batch_get_vdot(gp_batch* batch, double* vdot);
 /* dynamics_continuous(state, tau) (dynamics.rs:322): vdot for every instance at
  * the current state. tau NULL = held torques. Host arrays, synchronous. */
 int gp_dynamics(gp_batch* batch, const double* tau, double* vdot);
+
+/* mass_matrix(state, bodies_to_root) (mechanism.rs:637-696): the joint-space
+ * inertia M(q) of every instance at the batch's current state, by the
+ * composite-rigid-body algorithm on the GPU (FP64). M receives, instance after
+ * instance, an nv_i x nv_i row-major block (sum of nv_i^2 values; a floating
+ * joint's rows and columns are [w; v] of its body twist, as the reference
+ * orders them). Host array, synchronous. */
+int gp_mass_matrix(gp_batch* batch, double* M);
+
+/* dynamics_bias(state, ...) (dynamics.rs:233-251): c(q, v) - tau_c of every
+ * instance at the batch's current state, the joint torques of the
+ * velocity-product and gravity terms minus those of the contact wrenches
+ * (contact_dynamics, contact.rs:97-129), by recursive Newton-Euler with zero
+ * joint accelerations on the GPU (FP64). Prismatic springs are not included
+ * (dynamics_continuous adds them to tau, dynamics.rs:298-315); spring
+ * contacts act with their current registration, which this call does not
+ * change. c has nv_total values in the v flattening. Host array,
+ * synchronous. */
+int gp_dynamics_bias(gp_batch* batch, double* c);
 
 /* simulate(state, final_time, dt, control_fn, integrator) (simulate.rs:87).
  * The loop is the reference's float-accumulated `while t < final_time`
@@ -355,8 +380,12 @@ int gp_server_completed(gp_server* server, int64_t* ids, int64_t max_ids, int64_
  * instance produced a non-finite value. */
 int gp_server_result(gp_server* server, int64_t request_id, double* q, double* v, double* latency_s,
                      int32_t* fault);
-/* [rounds, env_steps, busy_seconds, admitted, completed, pools, pool_bytes,
- *  budget_bytes] */
+/* 10 values: [rounds, env_steps, busy_seconds, admitted, completed, pools,
+ *  pool_bytes, budget_bytes, admit_seconds, retire_seconds]; busy_seconds is
+ *  the host time from each round's launch to its results, admit_seconds the
+ *  host time spent admitting (packing, one transfer, one launch per round),
+ *  retire_seconds the time spent packing and copying back finished slots
+ *  (inside busy_seconds). */
 int gp_server_stats(const gp_server* server, double* stats);
 
 #ifdef __cplusplus

@@ -279,21 +279,58 @@ int oracle_contact_distances(const gp_model_desc* d, const double* q, double* di
 }
 
 // Operation counts of one step of the ABA formulation with the given
-// integrator at state (q, v): [add, mul, div, sqrt, trig, pow].
+// integrator at state (q, v): [add, mul, div, sqrt, trig, pow] of the
+// operations run in FP32 by the mixed-precision kernels, then the same six of
+// those run in FP64 (F64Scope: the floating pose, contact distances, u, the
+// joint-space solves, and the integrator's update of q and v).
 int oracle_opcount(const gp_model_desc* d, const double* q, const double* v, const double* tau,
                    int integrator, uint64_t* counts) {
+  OpCounter::loading = true;  // model and state values are data, never structural zeros
   Model<Cnt> M = load_model<Cnt>(*d);
-  AbaModel<Cnt> A = make_aba_model(M);
   std::vector<Cnt> qt(M.nq), vt(M.nv), tt(M.nv), qo(M.nq), vo(M.nv), ao(M.nv);
   for (int k = 0; k < M.nq; ++k) qt[k] = Cnt(q[k]);
   for (int k = 0; k < M.nv; ++k) { vt[k] = Cnt(v[k]); tt[k] = Cnt(tau ? tau[k] : 0.0); }
+  const Cnt dt(0.001);
+  OpCounter::loading = false;
+  AbaModel<Cnt> A = make_aba_model(M);
   auto dyn = [&](const Model<Cnt>&, const Cnt* qq, const Cnt* vv, const Cnt* ta, Cnt* out) {
-    return aba_dynamics(A, qq, vv, ta, out);
+    --OpCounter::f64;  // the dynamics marks its own FP64 parts
+    const int r = aba_dynamics(A, qq, vv, ta, out);
+    ++OpCounter::f64;
+    return r;
   };
   OpCounter::reset();
-  int st = step_integrate(M, integrator, qt.data(), vt.data(), tt.data(), Cnt(0.001), qo.data(), vo.data(), ao.data(), dyn);
+  int st;
+  {
+    F64Scope f64;  // the integrator's update of q and v
+    st = step_integrate(M, integrator, qt.data(), vt.data(), tt.data(), dt, qo.data(), vo.data(), ao.data(), dyn);
+  }
+  if (M.n == 1 && A.kind[0] == K_FLOAT) {
+    // A lone free body's D is its rigid-body inertia, the same at every step:
+    // its factorisation is loop-invariant work, not part of a step.
+    S6<Cnt> I6;
+    M3<Cnt> cx = skew(A.c[0]);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        I6.a[a][b] = A.J[0].m[a][b];
+        I6.a[a][b + 3] = cx.m[a][b];
+        I6.a[a + 3][b] = cx.m[b][a];
+        I6.a[a + 3][b + 3] = a == b ? A.m[0] : Cnt(0.0);
+      }
+    const uint64_t a0 = OpCounter::add64, m0 = OpCounter::mul64, d0 = OpCounter::div64, s0 = OpCounter::sqrt64;
+    Cnt L[6][6];
+    {
+      F64Scope f64;
+      chol6_factor(I6, L);
+    }
+    const uint64_t da = OpCounter::add64 - a0, dm = OpCounter::mul64 - m0, dd = OpCounter::div64 - d0,
+                   ds = OpCounter::sqrt64 - s0;
+    OpCounter::add64 = a0 - da; OpCounter::mul64 = m0 - dm; OpCounter::div64 = d0 - dd; OpCounter::sqrt64 = s0 - ds;
+  }
   counts[0] = OpCounter::add; counts[1] = OpCounter::mul; counts[2] = OpCounter::div;
   counts[3] = OpCounter::sqrt_; counts[4] = OpCounter::trig; counts[5] = OpCounter::pow_;
+  counts[6] = OpCounter::add64; counts[7] = OpCounter::mul64; counts[8] = OpCounter::div64;
+  counts[9] = OpCounter::sqrt64; counts[10] = OpCounter::trig64; counts[11] = OpCounter::pow64;
   return st;
 }
 

@@ -37,6 +37,20 @@
 
 namespace oracle {
 
+// Operation counters of the op-counting scalar `Cnt` (defined at the end).
+struct OpCounter {
+  static inline thread_local uint64_t add = 0, mul = 0, div = 0, sqrt_ = 0, trig = 0, pow_ = 0;
+  static inline thread_local uint64_t add64 = 0, mul64 = 0, div64 = 0, sqrt64 = 0, trig64 = 0, pow64 = 0;
+  static inline thread_local int f64 = 0;       // > 0 inside an F64Scope
+  static inline thread_local bool loading = false;
+  static void reset() { add = mul = div = sqrt_ = trig = pow_ = 0; add64 = mul64 = div64 = sqrt64 = trig64 = pow64 = 0; }
+  static void count(uint64_t& c32, uint64_t& c64) { ++(f64 > 0 ? c64 : c32); }
+};
+struct F64Scope {
+  F64Scope() { ++OpCounter::f64; }
+  ~F64Scope() { --OpCounter::f64; }
+};
+
 using std::size_t;
 
 // ---------------------------------------------------------------- small algebra
@@ -1304,14 +1318,15 @@ template <class T> S6<T> xtix(const M3<T>& E, const V3<T>& r, const S6<T>& I) {
   return out;
 }
 
-// Cholesky solve of a 6x6 SPD system (the floating joint's D)
-template <class T> int chol6_solve(const S6<T>& A, const V6<T>& b, V6<T>& x) {
+// Cholesky factor of a 6x6 SPD system (the floating joint's D); false if not SPD.
+template <class T> bool chol6_factor(const S6<T>& A, T (&L)[6][6]) {
   using std::sqrt;
-  T L[6][6] = {};
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) L[i][j] = T(0);
   for (int j = 0; j < 6; ++j) {
     T s = A.a[j][j];
     for (int k = 0; k < j; ++k) s = s - L[j][k] * L[j][k];
-    if (!(s > T(0))) return 1;
+    if (!(s > T(0))) return false;
     L[j][j] = sqrt(s);
     for (int i = j + 1; i < 6; ++i) {
       T t = A.a[i][j];
@@ -1319,6 +1334,12 @@ template <class T> int chol6_solve(const S6<T>& A, const V6<T>& b, V6<T>& x) {
       L[i][j] = t / L[j][j];
     }
   }
+  return true;
+}
+// Cholesky solve of a 6x6 SPD system (the floating joint's D)
+template <class T> int chol6_solve(const S6<T>& A, const V6<T>& b, V6<T>& x) {
+  T L[6][6];
+  if (!chol6_factor(A, L)) return 1;
   T y[6];
   for (int i = 0; i < 6; ++i) {
     T s = b.a[i];
@@ -1346,6 +1367,8 @@ int aba_dynamics(const AbaModel<T>& A, const T* q, const T* v, const T* tau, T* 
   std::vector<S6<T>> U(n);
   std::vector<S6<T>> Dm(n);
   std::vector<V6<T>> uu(n);
+  bool any_contact = false;
+  for (const auto& cs : A.contacts) any_contact = any_contact || !cs.empty();
   // pass 1: kinematics, velocities, bias terms
   for (int i = 0; i < n; ++i) {
     const T* qi = q + A.qoff[i];
@@ -1366,6 +1389,7 @@ int aba_dynamics(const AbaModel<T>& A, const T* q, const T* v, const T* tau, T* 
         vj.l.z = vi[0];
         break;
       case K_FLOAT: {
+        F64Scope f64;  // the mixed-precision kernels keep the floating pose in FP64
         Q<T> qq(qi[0], qi[1], qi[2], qi[3]);
         M3<T> Rq = qmat(qq);
         p = p + A.Rt[i] * V3<T>(qi[4], qi[5], qi[6]);
@@ -1385,8 +1409,11 @@ int aba_dynamics(const AbaModel<T>& A, const T* q, const T* v, const T* tau, T* 
     cb[i].a = cross(vel[i].a, vj.a);
     cb[i].l = cross(vel[i].a, vj.l) + cross(vel[i].l, vj.a);
     // world pose (for contacts)
-    if (pa < 0) { Rw[i] = R; tw[i] = p; }
-    else { Rw[i] = Rw[pa] * R; tw[i] = tw[pa] + Rw[pa] * p; }
+    if (any_contact) {
+      F64Scope f64;  // contact distances are FP64 in every kernel
+      if (pa < 0) { Rw[i] = R; tw[i] = p; }
+      else { Rw[i] = Rw[pa] * R; tw[i] = tw[pa] + Rw[pa] * p; }
+    }
     // rigid-body inertia and bias force v x* (I v) - f_ext
     S6<T>& I6 = IA[i];
     M3<T> cx = skew(A.c[i]);
@@ -1404,11 +1431,19 @@ int aba_dynamics(const AbaModel<T>& A, const T* q, const T* v, const T* tau, T* 
     f.l = cross(vel[i].a, h.l);
     // contacts: world-frame force from the half-spaces, applied in body coords
     for (const Contact<T>& cp : A.contacts[i]) {
-      V3<T> pw = Rw[i] * cp.p + tw[i];
+      V3<T> pw;
+      {
+        F64Scope f64;
+        pw = Rw[i] * cp.p + tw[i];
+      }
       V3<T> vb = vel[i].l + cross(vel[i].a, cp.p);
       V3<T> vw = Rw[i] * vb;
       for (const HalfSpace<T>& hs : A.halfspaces) {
-        T d = dot(pw - hs.point, hs.normal);
+        T d;
+        {
+          F64Scope f64;
+          d = dot(pw - hs.point, hs.normal);
+        }
         if (!(d <= T(kInsideMargin))) continue;
         V3<T> Fw = contact_force(-d, hs.normal, vw, cp.k, T(kHalfspaceK), hs.alpha, hs.mu);
         V3<T> Fb = Rw[i].transpose() * Fw;
@@ -1432,10 +1467,13 @@ int aba_dynamics(const AbaModel<T>& A, const T* q, const T* v, const T* tau, T* 
       for (int a = 0; a < 6; ++a) Ui.a[a] = I6.a[a][s];
       T D = Ui.a[s];
       if (!(D > T(0))) return 1;
-      T tq = tau[A.voff[i]];
-      if (k == K_PRI && !(A.sk[i] == T(0)))
-        tq = tq + (-A.sk[i] * (q[A.qoff[i]] - A.sl[i]));
-      T u = tq - p6.a[s];
+      T tq = tau[A.voff[i]], u;
+      {
+        F64Scope f64;  // u and the joint-space inertia's solve are FP64 in every kernel
+        if (k == K_PRI && !(A.sk[i] == T(0)))
+          tq = tq + (-A.sk[i] * (q[A.qoff[i]] - A.sl[i]));
+        u = tq - p6.a[s];
+      }
       U[i].a[0][0] = D;  // keep D, U and u for pass 3
       for (int a = 0; a < 6; ++a) U[i].a[1][a] = Ui.a[a];
       uu[i].a[0] = u;
@@ -1451,6 +1489,7 @@ int aba_dynamics(const AbaModel<T>& A, const T* q, const T* v, const T* tau, T* 
       }
     } else if (k == K_FLOAT) {
       Dm[i] = I6;
+      F64Scope f64;
       for (int a = 0; a < 6; ++a) uu[i].a[a] = tau[A.voff[i] + a] - p6.a[a];
       // Ia = IA - IA IA^-1 IA = 0; pa = pA + 0 c + IA IA^-1 u = pA + u = tau
       for (int a = 0; a < 6; ++a) { for (int b = 0; b < 6; ++b) Ia.a[a][b] = T(0); pa6.a[a] = p6.a[a] + uu[i].a[a]; }
@@ -1482,11 +1521,16 @@ int aba_dynamics(const AbaModel<T>& A, const T* q, const T* v, const T* tau, T* 
       V6<T> a6 = sv_to6(ap);
       T Ua(0);
       for (int a = 0; a < 6; ++a) Ua = Ua + U[i].a[1][a] * a6.a[a];
-      T qdd = (uu[i].a[0] - Ua) / U[i].a[0][0];
+      T qdd;
+      {
+        F64Scope f64;
+        qdd = (uu[i].a[0] - Ua) / U[i].a[0][0];
+      }
       vdot[A.voff[i]] = qdd;
       a6.a[s] = a6.a[s] + qdd;
       aa[i] = v6_to_sv(a6);
     } else if (k == K_FLOAT) {
+      F64Scope f64;  // the floating joint's 6x6 solve
       V6<T> a6 = sv_to6(ap);
       V6<T> rhs;
       for (int a = 0; a < 6; ++a) {
@@ -1506,31 +1550,61 @@ int aba_dynamics(const AbaModel<T>& A, const T* q, const T* v, const T* tau, T* 
 }
 
 // Scalar that counts arithmetic operations (algorithmic flop count per step).
-struct OpCounter {
-  static inline thread_local uint64_t add = 0, mul = 0, div = 0, sqrt_ = 0, trig = 0, pow_ = 0;
-  static void reset() { add = mul = div = sqrt_ = trig = pow_ = 0; }
-};
+// An operation with a structural zero or one of the formulation (a literal
+// T(0) / T(1): the zero blocks of a rigid-body inertia, the identity of a
+// motion subspace, a velocity's unused components) is not counted, since an
+// implementation specialised to the formulation's structure never performs
+// it; values loaded from the model or state (OpCounter::loading) count even
+// when they happen to be 0 or 1. Operations run in FP64 by the mixed-precision
+// kernels (F64Scope) are counted apart from the rest.
 struct Cnt {
   double v;
-  Cnt() : v(0) {}
-  Cnt(double x) : v(x) {}
+  signed char s;  // 0: a value; 1: structural zero; 2: structural one
+  Cnt() : v(0), s(1) {}
+  Cnt(double x) : v(x), s(OpCounter::loading ? 0 : (x == 0.0 ? 1 : (x == 1.0 ? 2 : 0))) {}
+  static Cnt val(double x) { Cnt c; c.v = x; c.s = 0; return c; }
   explicit operator double() const { return v; }
+  bool zero() const { return s == 1; }
+  bool one() const { return s == 2; }
 };
-inline Cnt operator+(Cnt a, Cnt b) { ++OpCounter::add; return Cnt(a.v + b.v); }
-inline Cnt operator-(Cnt a, Cnt b) { ++OpCounter::add; return Cnt(a.v - b.v); }
-inline Cnt operator-(Cnt a) { return Cnt(-a.v); }
-inline Cnt operator*(Cnt a, Cnt b) { ++OpCounter::mul; return Cnt(a.v * b.v); }
-inline Cnt operator/(Cnt a, Cnt b) { ++OpCounter::div; return Cnt(a.v / b.v); }
+inline Cnt operator+(Cnt a, Cnt b) {
+  if (a.zero()) return b;
+  if (b.zero()) return a;
+  OpCounter::count(OpCounter::add, OpCounter::add64);
+  return Cnt::val(a.v + b.v);
+}
+inline Cnt operator-(Cnt a) { Cnt r = a; r.v = -a.v; if (a.one()) r.s = 0; return r; }
+inline Cnt operator-(Cnt a, Cnt b) {
+  if (b.zero()) return a;
+  if (a.zero()) return -b;
+  OpCounter::count(OpCounter::add, OpCounter::add64);
+  return Cnt::val(a.v - b.v);
+}
+inline Cnt operator*(Cnt a, Cnt b) {
+  if (a.zero() || b.zero()) return Cnt();
+  if (a.one()) return b;
+  if (b.one()) return a;
+  OpCounter::count(OpCounter::mul, OpCounter::mul64);
+  return Cnt::val(a.v * b.v);
+}
+inline Cnt operator/(Cnt a, Cnt b) {
+  if (a.zero()) return Cnt();
+  if (b.one()) return a;
+  OpCounter::count(OpCounter::div, OpCounter::div64);
+  return Cnt::val(a.v / b.v);
+}
+inline Cnt& operator+=(Cnt& a, Cnt b) { return a = a + b; }
+inline Cnt& operator-=(Cnt& a, Cnt b) { return a = a - b; }
 inline bool operator<(Cnt a, Cnt b) { return a.v < b.v; }
 inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
 inline bool operator<=(Cnt a, Cnt b) { return a.v <= b.v; }
 inline bool operator==(Cnt a, Cnt b) { return a.v == b.v; }
 inline bool operator!=(Cnt a, Cnt b) { return a.v != b.v; }
-inline Cnt sqrt(Cnt a) { ++OpCounter::sqrt_; return Cnt(std::sqrt(a.v)); }
-inline Cnt sin(Cnt a) { ++OpCounter::trig; return Cnt(std::sin(a.v)); }
-inline Cnt cos(Cnt a) { ++OpCounter::trig; return Cnt(std::cos(a.v)); }
-inline Cnt fabs(Cnt a) { return Cnt(std::fabs(a.v)); }
-inline Cnt fmax(Cnt a, Cnt b) { return Cnt(std::fmax(a.v, b.v)); }
-inline Cnt pow(Cnt a, Cnt b) { ++OpCounter::pow_; return Cnt(std::pow(a.v, b.v)); }
+inline Cnt sqrt(Cnt a) { OpCounter::count(OpCounter::sqrt_, OpCounter::sqrt64); return Cnt::val(std::sqrt(a.v)); }
+inline Cnt sin(Cnt a) { OpCounter::count(OpCounter::trig, OpCounter::trig64); return Cnt::val(std::sin(a.v)); }
+inline Cnt cos(Cnt a) { OpCounter::count(OpCounter::trig, OpCounter::trig64); return Cnt::val(std::cos(a.v)); }
+inline Cnt fabs(Cnt a) { Cnt r = a; r.v = std::fabs(a.v); return r; }
+inline Cnt fmax(Cnt a, Cnt b) { return Cnt::val(std::fmax(a.v, b.v)); }
+inline Cnt pow(Cnt a, Cnt b) { OpCounter::count(OpCounter::pow_, OpCounter::pow64); return Cnt::val(std::pow(a.v, b.v)); }
 
 }  // namespace oracle

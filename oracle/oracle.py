@@ -176,6 +176,9 @@ class Oracle:
         nq, nv = self.dims(d)
         q = np.ascontiguousarray(q, float); v = np.ascontiguousarray(v, float)
         t = np.zeros(nv) if tau is None else np.ascontiguousarray(tau, float)
-        c = (C.c_uint64 * 6)()
+        c = (C.c_uint64 * 12)()
         self.lib.oracle_opcount(self._desc(d), _dp(q), _dp(v), _dp(t), integrator, c)
-        return dict(add=c[0], mul=c[1], div=c[2], sqrt=c[3], trig=c[4], pow=c[5])
+        names = ("add", "mul", "div", "sqrt", "trig", "pow")
+        out = {k: c[i] for i, k in enumerate(names)}
+        out.update({k + "64": c[6 + i] for i, k in enumerate(names)})
+        return out

@@ -123,7 +123,7 @@ def test_batch_without_device_reports_device_error(gp):
     assert e.value.code == gp.abi.GP_ERR_DEVICE
 
 
-def test_oracle_library_loads(oracle):
+def test_oracle_library_loads(gp, oracle):
     from gorilla_physics_amd import workloads
     assert oracle.dims(workloads.cube_desc()) == (7, 6)
 

@@ -40,10 +40,28 @@ def _check_line(out, instances, dtype):
 def test_arms_line_with_roofline(gp):
     print("case bench arms 4096", flush=True)
     out = _bench("--instances", "4096", "--steps", "200", "--warmup", "200", "--no-serve", "--no-cpu-baseline")
-    _check_line(out, 4096, "f32+f64")
+    _check_line(out, 4096, "f64")
+    for r in (out["roofline"], out["mixed_precision"]["roofline"]):
+        assert set(r) >= {"bound", "achieved", "peak", "unit", "frac", "traffic"}
+        assert r["unit"] == "TFLOP/s" and 0 < r["frac"] < 1 and abs(r["achieved"] / r["peak"] - r["frac"]) < 1e-12
+    # FP64 kernel: rated at the FP64 peak; the mix between the two peaks
+    assert out["roofline"]["peak"] == 78.6
+    assert 78.6 < out["mixed_precision"]["roofline"]["peak"] < 157.3
+    assert out["valid_value"] <= out["value"]
+
+
+@pytest.mark.gpu
+def test_timed_call_replays_prepared_graphs(gp):
+    """The driver's shape (20 timed steps after 5 warmup steps): the graphs
+    of both are built before either runs, so the HIP-event time of the timed
+    launch is the kernel's, not a capture's."""
+    print("case bench arms 4096 steps 20 warmup 5", flush=True)
+    out = _bench("--instances", "4096", "--steps", "20", "--warmup", "5", "--no-serve", "--no-cpu-baseline",
+                 "--no-compare")
     r = out["roofline"]
-    assert set(r) >= {"bound", "achieved", "peak", "unit", "frac", "traffic"}
-    assert r["unit"] == "TFLOP/s" and 0 < r["frac"] < 1 and abs(r["achieved"] / r["peak"] - r["frac"]) < 1e-12
+    assert r["launches"] == 1 and r["steps_per_launch"] == 20
+    # wall-clock per step includes one graph launch and a synchronisation, not an instantiation
+    assert r["launch_ms"] <= out["ms_per_step"] * 20
 
 
 @pytest.mark.gpu

@@ -74,6 +74,21 @@ def test_served_fp64_results_match_the_oracle(gp, oracle):
         assert lat > 0
 
 
+def test_more_pools_than_one_launch_table(gp):
+    """One slot per pool, every request arriving at once: more pools than one
+    grouped-launch table holds (32), so each round admits, steps and retires
+    through several tables. The results equal those of the same requests
+    served in a few large pools, bit for bit."""
+    reqs = _requests(gp, 48, seed=5)
+    print(f"case serving one slot per pool requests={len(reqs)}", flush=True)
+    out_small, s = _serve(gp, reqs, 0, slots=1)
+    assert s.stats()["pools"] > 32
+    out_big, _ = _serve(gp, reqs, 0, slots=64)
+    for (qa, va, _, fa), (qb, vb, _, fb) in zip(out_small, out_big):
+        assert not fa and not fb
+        assert np.array_equal(qa, qb) and np.array_equal(va, vb)
+
+
 def test_memory_budget_admits_and_evicts(gp):
     """With room for a single pool, requests of two topologies still all
     complete: the second topology waits until the first pool is idle and

@@ -182,6 +182,39 @@ def test_cart_and_cart_pole_rk4(gp, oracle):  # simulate.rs:177,218
     assert abs(vs[-1, 0] - acc * 20.0) < 1e-4 and abs(qs[-1, 1] - th) < 1e-5
 
 
+def test_rk2_one_step_closed_form(gp, oracle):
+    """runge_kutta_2 (integrators.rs:177-193) on a sprung prismatic slider
+    (qdd = -k (q - l) / m, a closed form; gravity is normal to the rail): the
+    second evaluation is at euler_step(q0, v0, f1, dt/2) = (q0 + dt/2 v0,
+    v0 + dt/2 f1), and the result is euler_step(q0, v0, f2, dt): q' = q0 + dt
+    v0 (the start velocity, not the midpoint's), v' = v0 + dt f2. A midpoint
+    position update, or f1 in place of f2, misses by orders more than the
+    tolerance. runge_kutta_4 (integrators.rs:195-225) lands on the same
+    point here: its stages sit at q0, q0 + dt/2 v0 (twice) and q0 + dt v0, and
+    (f1 + 4 f2 + f4) / 6 of a force linear in q is f2."""
+    from gorilla_physics_amd import abi
+    from gorilla_physics_amd.mechanism import (JointSpring, MechanismState, PrismaticJoint, RigidBody,
+                                               SpatialInertia, Transform3D, WORLD_FRAME)
+    m, k, l = 2.0, 300.0, 0.25
+    body = RigidBody(SpatialInertia("slider", np.diag([0.1, 0.2, 0.3]), [0.05, 0.0, 0.02], m))
+    st = MechanismState([PrismaticJoint.new_with_spring(Transform3D.identity("slider", WORLD_FRAME), [1.0, 0.0, 0.0],
+                                                        JointSpring(k, l))], [body])
+    d = st.model_desc()
+    q0, v0, dt = 0.7, -1.3, 0.01
+    f = lambda q: -k * (q - l) / m
+    f1 = f(q0)
+    f2 = f(q0 + dt / 2 * v0)
+    q_ref, v_ref = q0 + dt * v0, v0 + dt * f2
+    q1, v1, a1 = oracle.step(d, [q0], [v0], [0.0], dt, abi.GP_RUNGE_KUTTA_2)
+    assert abs(a1[0] - f1) <= 1e-13 * abs(f1)
+    assert abs(q1[0] - q_ref) <= 1e-15 and abs(v1[0] - v_ref) <= 1e-13 * abs(v_ref)
+    wrong_q = q0 + dt * (v0 + dt / 2 * f1)  # midpoint position
+    wrong_v = v0 + dt * f1                  # first-stage slope
+    assert abs(q1[0] - wrong_q) > 1e3 * 1e-15 and abs(v1[0] - wrong_v) > 1e3 * 1e-13 * abs(v_ref)
+    q4, v4, _ = oracle.step(d, [q0], [v0], [0.0], dt, abi.GP_RUNGE_KUTTA_4)
+    assert abs(q4[0] - q_ref) <= 1e-15 and abs(v4[0] - v_ref) <= 1e-13 * abs(v_ref)
+
+
 def test_cart_pole_energy_rk4(gp, oracle):  # energy.rs:130
     from gorilla_physics_amd import builders
     cp = builders.build_cart_pole(1.0, 1.0, np.diag([0, 1 / 12, 1 / 12]), np.diag([1.0, 1.0, 0]), [0, 0, 0],
@@ -688,18 +721,52 @@ def test_mass_matrix_spd_and_equation_of_motion(gp, oracle):
         assert np.abs(M @ a + c - t).max() < 1e-9 * max(1.0, np.abs(t).max(), np.abs(c).max())
 
 
+def _opcount_split(oracle, d, q, v, tau, integrator=0):
+    c = oracle.opcount(d, q, v, tau, integrator)
+    return (sum(x for k, x in c.items() if not k.endswith("64")), sum(x for k, x in c.items() if k.endswith("64")))
+
+
 def test_work_table_matches_opcount(gp, oracle):
     """The algorithmic flop counts used by bench.py's roofline equal the
-    op-counting instantiation of the CPU articulated-body algorithm."""
+    op-counting instantiation of the CPU articulated-body algorithm, summed
+    over every instance of each workload."""
     from gorilla_physics_amd import workloads
     table = json.load(open(os.path.join(os.path.dirname(__file__), "..", "gorilla-physics_amd",
                                         "algorithmic_work.json")))
-    w = workloads.arms(n=2)
-    c = oracle.opcount(w["models"][0], w["q"][:7], w["v"][:7], w["tau"][:7])
-    assert sum(c.values()) == table["arms"]["flops_per_env_step"]
-    w = workloads.acrobot(n=2)
-    c = oracle.opcount(w["models"][0], w["q"][:2], w["v"][:2], w["tau"][:2])
-    assert sum(c.values()) == table["acrobot"]["flops_per_env_step"]
+    for name, n in (("arms", 65536), ("acrobot", 1024), ("cubes", 16384), ("mixed", 32768), ("trees", 16384),
+                    ("so101", 65536)):
+        w = getattr(workloads, name)(n=n if name in ("mixed", "trees") else 4)
+        t32 = t64 = 0
+        qo = vo = 0
+        for d, c in zip(w["models"], w["counts"]):
+            nq, nv = oracle.dims(d)
+            f32, f64 = _opcount_split(oracle, d, w["q"][qo:qo + nq], w["v"][vo:vo + nv], w["tau"][vo:vo + nv],
+                                      w.get("integrator", 0))
+            t32 += f32 * c
+            t64 += f64 * c
+            qo += nq * c
+            vo += nv * c
+        if name not in ("mixed", "trees"):  # every instance of these has the same count: scale the sample
+            t32, t64 = t32 * n // sum(w["counts"]), t64 * n // sum(w["counts"])
+        assert (t32, t64) == (table[name]["f32_per_batch_step"], table[name]["f64_per_batch_step"]), name
+        assert table[name]["instances"] == n
+
+
+def test_opcount_counts_structure_not_data(gp, oracle):
+    """Model values that happen to be 0 or 1 still cost operations; only the
+    formulation's structural zeros are free. A 1-link revolute pendulum whose
+    tree rotation is the identity (exact zeros and ones as data) counts the
+    same as one with a generic rotation."""
+    from gorilla_physics_amd import abi
+    from gorilla_physics_amd.mechanism import Isometry3, ModelDesc, SpatialInertia, UnitQuaternion, body_desc
+    inertia = SpatialInertia("rod", np.diag([0.1, 0.2, 0.3]), [0.1, 0.0, 0.0], 1.0)
+    ident = ModelDesc([body_desc(-1, abi.GP_JOINT_REVOLUTE, [0, 0, 1], Isometry3.identity(), inertia)], [], [])
+    rot = Isometry3(UnitQuaternion.from_axis_angle(np.array([1.0, 2.0, 3.0]) / np.sqrt(14.0), 0.3),
+                    np.array([0.1, -0.2, 0.3]))
+    gen = ModelDesc([body_desc(-1, abi.GP_JOINT_REVOLUTE, [0, 0, 1], rot, inertia)], [], [])
+    a = oracle.opcount(ident, [0.4], [0.2], [0.1])
+    b = oracle.opcount(gen, [0.4], [0.2], [0.1])
+    assert a == b and sum(a.values()) > 0
 
 
 # ------------------------------------------------------------------ spring contacts (contact.rs:74-186)
