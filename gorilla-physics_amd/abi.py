@@ -146,6 +146,7 @@ SIGNATURES = {
     "gp_batch_set_tau": (C.c_int, [_P, _D]),
     "gp_step": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, _P]),
     "gp_step_prepare": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, _P]),
+    "gp_step_noalloc": (C.c_int, [_P, C.c_double, C.c_int32, C.c_int32, _P]),
     "gp_mass_matrix": (C.c_int, [_P, _D]),
     "gp_dynamics_bias": (C.c_int, [_P, _D]),
     "gp_batch_sync": (C.c_int, [_P]),

@@ -588,6 +588,39 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
   return GP_OK;
 }
 
+// ------------------------------------------------------------------ no-allocation step
+// gp_step_noalloc: the kernels' scratch (the runtime-topology and
+// velocity-stepping kernels' per-lane working set) comes from one file-scope
+// device array per GPU instead of the buckets' lazily allocated buffers; no
+// graph, no grouped launch table, no allocation of any kind. Every call on a
+// GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+// other threads) race on it.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this GPU's `scratch` (host code may not use the
+// array's name as an address), looked up once per device.
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(GP_ERR_DEVICE, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(GP_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return GP_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* gp_last_error(void) { return g_error.c_str(); }
@@ -1011,6 +1044,26 @@ int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* 
   return GP_OK;
 }
 
+
+int gp_step_noalloc(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* stream) {
+  int st = check_step_args(bt, dt, integrator, n_steps);
+  if (st) return st;
+  size_t need = 0;  // the largest working set among the buckets that need one
+  for (const Bucket& b : bt->buckets)
+    if (!bucket_is_fast(bt->opts, b, integrator)) need = std::max(need, bucket_scratch_doubles(b) * sizeof(double));
+  if (need > kNoAllocScratchBytes)
+    return fail(GP_ERR_UNSUPPORTED,
+                "gp_step_noalloc: a bucket needs %zu bytes of scratch, more than the %zu of the static array; "
+                "use gp_step or fewer instances per batch",
+                need, kNoAllocScratchBytes);
+  double* scr = nullptr;
+  if ((st = noalloc_scratch(&scr))) return st;
+  int64_t launches = 0;
+  hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
+  if ((st = batch_launch_all(bt, dt, integrator, n_steps, s, &launches, scr))) return st;
+  bt->timing_valid = false;
+  return GP_OK;
+}
 
 int gp_batch_sync(gp_batch* bt) {
   if (!bt) return fail(GP_ERR_INVALID, "null batch");

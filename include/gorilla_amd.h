@@ -255,6 +255,19 @@ int gp_step(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, voi
  * and one for a shorter last chunk), so that later gp_step calls only launch.
  * Nothing is stepped. No reference counterpart (setup of the batched launch). */
 int gp_step_prepare(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, void* stream);
+/* gp_step without any allocation, for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Same results as
+ * gp_step, bit for bit; no hipGraph and no grouped launch (one launch per
+ * bucket and chunk). The runtime-topology and velocity-stepping kernels take
+ * their per-lane scratch from one static device array of the library
+ * (64 MiB per GPU, held until the process ends) instead of the batch's own
+ * buffers: a batch whose largest such bucket needs more returns
+ * GP_ERR_UNSUPPORTED and nothing runs. Every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other streams, other threads)
+ * race on it: serialise them. Not timed (gp_batch_last_timing refers to the
+ * last gp_step). */
+int gp_step_noalloc(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, void* stream);
 /* Block until the batch's work on its stream is done. */
 int gp_batch_sync(gp_batch* batch);
 
