@@ -24,6 +24,8 @@
 // never addresses: with both, no FP32 chain of up to 12 joints spills.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "../../include/gorilla_amd.h"
 #include "gp_device.h"
 #include "gp_internal.h"
@@ -167,44 +169,173 @@ struct Body1 {  // what a 1-DOF body keeps between the sweeps
   int rev;  // 1 revolute, 0 prismatic
 };
 
+// Lane-private LDS tier of the tree parameters that do not stay in
+// registers. Per body (from the first uncached one on): its tree rotation as a
+// unit quaternion and its translation, 7 values, the rotation rebuilt where a
+// sweep uses it (24 operations in place of 9 loads); then, while room is left,
+// the rigid inertias (10 values) of the first of those bodies. One row of
+// `pitch` values per lane (lane-major, so every access is the lane's base
+// register plus an immediate offset; an odd pitch spreads a wave's 64-bit
+// accesses over all banks), written once per launch; a lane reads and writes
+// only its own row, so no barrier is needed. Budget: kLdsLaneBytes per lane,
+// the CU's 160 KB over its 256 lanes at one wave per SIMD (the occupancy of
+// every kernel that has uncached bodies). Without it the FP64 7-joint chain
+// re-reads every body's 22 values in every sweep from beyond L2 (~110 MB per
+// step at 65,536 arms).
+constexpr int kLdsLaneBytes = 640;
+template <class Topo, class R, int CN> struct LdsTier {
+  static constexpr int lo = CN > Topo::first ? CN : Topo::first;  // first body in the tier
+  static constexpr int U = Topo::N - lo;
+  static constexpr int cap = kLdsLaneBytes / (int)sizeof(R) - 1;    // room for an odd pitch
+  static constexpr bool on = Topo::dof(Topo::first) == 1 && U > 0;  // chains and welded chains
+  static constexpr int NT = on ? (U < cap / 7 ? U : cap / 7) : 0;   // bodies [lo, lo + NT): transforms
+  static constexpr int NI0 = NT == U ? (cap - 7 * NT) / 10 : 0;
+  static constexpr int NI = NI0 < U ? NI0 : U;                     // bodies [lo, lo + NI): inertias
+  static constexpr int slots = 7 * NT + 10 * NI;
+  static constexpr int pitch = slots > 0 ? (slots | 1) : 0;
+  // the inertias of up to 4 more bodies [lo + NI, lo + NI + NR) in registers
+  // (a 7-joint FP64 chain then reads nothing from memory inside a step)
+  static constexpr int NR = (NT == U && U - NI <= 4) ? U - NI : 0;
+};
+
+// Unit quaternion (w, x, y, z) of a rotation matrix (row-major), by
+// Shepperd's method: the largest of 1 + trace and 1 + 2 R_kk - trace under
+// the square root.
+template <class T>
+__device__ __forceinline__ void rot_to_quat(const m3<T>& M, T* o) {
+  const T m00 = M.a[0], m01 = M.a[1], m02 = M.a[2], m10 = M.a[3], m11 = M.a[4], m12 = M.a[5], m20 = M.a[6],
+          m21 = M.a[7], m22 = M.a[8];
+  const T tr = m00 + m11 + m22;
+  T w, x, y, z;
+  if (tr >= m00 && tr >= m11 && tr >= m22) {
+    const T r = sqrt(T(1) + tr), h = T(0.5) / r;
+    w = T(0.5) * r; x = (m21 - m12) * h; y = (m02 - m20) * h; z = (m10 - m01) * h;
+  } else if (m00 >= m11 && m00 >= m22) {
+    const T r = sqrt(T(1) + m00 - m11 - m22), h = T(0.5) / r;
+    x = T(0.5) * r; w = (m21 - m12) * h; y = (m01 + m10) * h; z = (m02 + m20) * h;
+  } else if (m11 >= m22) {
+    const T r = sqrt(T(1) + m11 - m00 - m22), h = T(0.5) / r;
+    y = T(0.5) * r; w = (m02 - m20) * h; x = (m01 + m10) * h; z = (m12 + m21) * h;
+  } else {
+    const T r = sqrt(T(1) + m22 - m00 - m11), h = T(0.5) / r;
+    z = T(0.5) * r; w = (m10 - m01) * h; x = (m02 + m20) * h; y = (m12 + m21) * h;
+  }
+  o[0] = w; o[1] = x; o[2] = y; o[3] = z;
+}
+
+template <class T>
+__device__ __forceinline__ m3<T> quat_to_rot(T w, T x, T y, T z) {
+  const T x2 = x + x, y2 = y + y, z2 = z + z;
+  const T xx = x * x2, yy = y * y2, zz = z * z2, xy = x * y2, xz = x * z2, yz = y * z2;
+  const T wx = w * x2, wy = w * y2, wz = w * z2;
+  return m3<T>{{T(1) - (yy + zz), xy - wz, xz + wy, xy + wz, T(1) - (xx + zz), yz - wx, xz - wy, yz + wx,
+                T(1) - (xx + yy)}};
+}
+
 // Per-body tree parameters of one instance ([NBF][N][Bp] in the bucket).
 // The first CN bodies' are loaded into registers once per launch (loads whose
 // values are never used, e.g. spring constants of an all-revolute chain, are
-// dropped); the others are read from the bucket's array where used (an L1/L2
-// hit). Caching pays while the ~22 values per body fit beside the sweep
-// state (see cached_bodies).
-template <class Topo, class R, int CN = 0>
+// dropped); with LDS, the next bodies' transforms and inertias are kept in the
+// lane's LDS column (LdsTier); the others are read from the bucket's array
+// where used (an L1/L2 hit at best). Caching pays while the ~22 values per
+// body fit beside the sweep state (see cached_bodies).
+template <class Topo, class R, int CN, bool LDS> struct TierOf {
+  static constexpr int NT = 0, NI = 0, NR = 0, lo = 0, pitch = 0;
+};
+template <class Topo, class R, int CN> struct TierOf<Topo, R, CN, true> {
+  using T = LdsTier<Topo, R, CN>;
+  static constexpr int NT = T::NT, NI = T::NI, NR = T::NR, lo = T::lo, pitch = T::pitch;
+};
+
+template <class Topo, class R, int CN = 0, bool LDS = false>
 struct FastParams {
+  using Tier = TierOf<Topo, R, CN, LDS>;
+  static constexpr int NT = Tier::NT, NI = Tier::NI, NR = Tier::NR, lo = Tier::lo, pitch = Tier::pitch;
   Soa<R> bp;
   int lane;
   int xbody;  // per-lane shift of the body index (a lane that runs one leg of a tree)
+  R* lds;     // this lane's LDS row (LdsTier): slot k at lds[k]
   R c[CN > 0 ? NBF : 1][CN > 0 ? CN : 1];
-  __device__ __forceinline__ FastParams(const R* p, int Bp, int lane_, int xbody_ = 0)
-      : bp(p, NBF * Topo::N, Bp), lane(lane_), xbody(xbody_) {
+  R ri[NR > 0 ? NR : 1][10];  // inertias of bodies lo + NI + j (J, m c, m)
+  __device__ __forceinline__ FastParams(const R* p, int Bp, int lane_, int xbody_ = 0, R* lds_base = nullptr)
+      : bp(p, NBF * Topo::N, Bp), lane(lane_), xbody(xbody_),
+        lds(lds_base ? lds_base + (int)threadIdx.x * pitch : nullptr) {
     if constexpr (CN > 0) {
 #pragma unroll
       for (int f = 0; f < NBF; ++f)
 #pragma unroll
         for (int i = 0; i < CN; ++i) c[f][i] = bp.ldx(f * Topo::N + i, lane, xbody);
     }
+    if constexpr (NT > 0) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        m3<R> M;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) M.a[k] = glob(BF_RT + k, lo + j);
+        R qt[4];
+        rot_to_quat(M, qt);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lds[7 * j + k] = qt[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) lds[7 * j + 4 + k] = glob(BF_PT + k, lo + j);
+      }
+#pragma unroll
+      for (int j = 0; j < NI; ++j)
+#pragma unroll
+        for (int k = 0; k < 10; ++k) lds[7 * NT + 10 * j + k] = glob(BF_J + k, lo + j);
+#pragma unroll
+      for (int j = 0; j < NR; ++j)
+#pragma unroll
+        for (int k = 0; k < 10; ++k) ri[j][k] = glob(BF_J + k, lo + NI + j);
+    }
+  }
+  __device__ __forceinline__ R glob(int f, int i) const { return bp.ldx(f * Topo::N + i, lane, xbody); }
+  // a compiler-only memory barrier: LDS values read after it are loaded again
+  __device__ __forceinline__ void sweep_fence() const {
+    if constexpr (NT > 0) asm volatile("" ::: "memory");
   }
   __device__ __forceinline__ R operator()(int f, int i) const {
     if (i < CN) return c[f][i < CN ? i : 0];
-    return bp.ldx(f * Topo::N + i, lane, xbody);
+    return glob(f, i);
   }
   __device__ __forceinline__ m3<R> Rt(int i) const {
+    if (NT > 0 && i >= lo && i < lo + NT && i >= CN) {
+      const int j = i - lo;
+      return quat_to_rot(lds[7 * j], lds[7 * j + 1], lds[7 * j + 2], lds[7 * j + 3]);
+    }
     m3<R> M;
 #pragma unroll
     for (int k = 0; k < 9; ++k) M.a[k] = (*this)(BF_RT + k, i);
     return M;
   }
-  __device__ __forceinline__ v3<R> pt(int i) const { return {(*this)(BF_PT, i), (*this)(BF_PT + 1, i), (*this)(BF_PT + 2, i)}; }
+  __device__ __forceinline__ v3<R> pt(int i) const {
+    if (NT > 0 && i >= lo && i < lo + NT && i >= CN) {
+      const int j = i - lo;
+      return {lds[7 * j + 4], lds[7 * j + 5], lds[7 * j + 6]};
+    }
+    return {(*this)(BF_PT, i), (*this)(BF_PT + 1, i), (*this)(BF_PT + 2, i)};
+  }
   __device__ __forceinline__ ai6<R> inertia(int i) const {
+    if (NI > 0 && i >= lo && i < lo + NI && i >= CN) {
+      const int b = 7 * NT + 10 * (i - lo);
+      auto at = [&](int k) { return lds[b + k]; };
+      return rigid(s3<R>{at(0), at(1), at(2), at(3), at(4), at(5)}, v3<R>{at(6), at(7), at(8)}, at(9));
+    }
+    if (NR > 0 && i >= lo + NI && i < lo + NI + NR) {
+      const R* e = ri[NR > 0 ? i - lo - NI : 0];
+      return rigid(s3<R>{e[0], e[1], e[2], e[3], e[4], e[5]}, v3<R>{e[6], e[7], e[8]}, e[9]);
+    }
     s3<R> J{(*this)(BF_J, i), (*this)(BF_J + 1, i), (*this)(BF_J + 2, i), (*this)(BF_J + 3, i), (*this)(BF_J + 4, i),
             (*this)(BF_J + 5, i)};
     return rigid(J, v3<R>{(*this)(BF_C, i), (*this)(BF_C + 1, i), (*this)(BF_C + 2, i)}, (*this)(BF_M, i));
   }
 };
+
+// Dynamic LDS of a launch (the tier's columns); GP_DYN_LDS names it.
+#ifndef GP_DYN_LDS
+extern __shared__ __align__(16) unsigned char gp_dyn_lds[];
+#define GP_DYN_LDS gp_dyn_lds
+#endif
 
 // Bodies whose parameters stay in registers for a whole launch without
 // spilling: every body of a chain of up to 9 joints (FP32 sweeps) or 4 (FP64),
@@ -225,6 +356,12 @@ template <class Topo, class R, bool ALLREV> constexpr int cached_bodies() {
   if (Topo::dof(Topo::first) != 1) return 0;
   if (Topo::N <= n_max) return Topo::N;
   return sizeof(R) == 4 ? (ALLREV ? GP_PCACHE_LONG : GP_PCACHE_LONG / 2) : 0;
+}
+
+// Bytes of dynamic LDS a launch of `block` lanes needs for the tier of a
+// kernel with this topology and precision.
+template <class Topo, class R, bool ALLREV> constexpr int lds_tier_bytes(int block) {
+  return LdsTier<Topo, R, cached_bodies<Topo, R, ALLREV>()>::pitch * (int)sizeof(R) * block;
 }
 
 // A lone floating body: its articulated inertia is its rigid inertia, the same
@@ -273,8 +410,8 @@ struct LaunchConst {
   bool cached = false;
 };
 
-template <class Topo, class R, int L, bool CONTACTS, int PC, bool TAU>
-__device__ __forceinline__ bool load_launch_const(const BucketArgs& a, const FastParams<Topo, R, PC>& P, int lane,
+template <class Topo, class R, int L, bool CONTACTS, class FP, bool TAU>
+__device__ __forceinline__ bool load_launch_const(const BucketArgs& a, const FP& P, int lane,
                                                   int sub, LaunchConst<Topo, L, CONTACTS, TAU>& lc) {
   using LC = LaunchConst<Topo, L, CONTACTS, TAU>;
   bool ok = true;
@@ -327,8 +464,8 @@ __device__ __forceinline__ bool load_launch_const(const BucketArgs& a, const Fas
 // RK2 / RK4). Torques are read from memory where used. ALLREV: every 1-DOF
 // joint of the bucket is revolute (no per-lane select). Iinv: the inverse
 // inertia of a lone free body (lone_free_body), else unused.
-template <class Topo, class R, int L, bool CONTACTS, bool ALLREV, bool SIE, int PC, class LC>
-__device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<Topo, R, PC>& P, int lane, int sub,
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV, bool SIE, class FP, class LC>
+__device__ __forceinline__ bool fast_step(const BucketArgs& a, const FP& P, int lane, int sub,
                                           double* q, double* v, double dt, bool last, const LC& lc, double* qdd_out) {
   constexpr int N = Topo::N;
   const int Bp = a.Bp;
@@ -345,6 +482,10 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
   m3<double> Rw[CONTACTS ? N : 1];
   v3<double> tw[CONTACTS ? N : 1];
   bool ok = true;
+  // the tier's values are re-read in every sweep, not kept in registers
+  // across sweeps (the compiler would otherwise hold every body's rebuilt
+  // rotation from pass 1 to pass 3 and spill)
+  P.sweep_fence();
   // ---- pass 1: kinematics, velocities, bias terms (root to leaves)
 #pragma unroll
   for (int i = Topo::first; i < N; ++i) {
@@ -460,6 +601,7 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
     }
     if (Topo::dof(i) == 1) b[i].pA = pA; else pA0 = pA;
   }
+  P.sweep_fence();
   // ---- pass 2: articulated inertias (leaves to root)
   ai6<R> acc[N];
   bool has_acc[N];
@@ -511,6 +653,7 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
       }
     }
   }
+  P.sweep_fence();
   // ---- pass 3: accelerations (root to leaves)
   sv<R> accel[N];
   sv<R> a0{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}};
@@ -583,7 +726,7 @@ __device__ __forceinline__ bool fast_step(const BucketArgs& a, const FastParams<
 // n_steps semi-implicit Euler steps per launch with the state in registers;
 // q and v accumulate in FP64. With `active`, lane l runs min(n_steps,
 // active[l]) steps (serving: instances with different horizons).
-template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV, bool LDS = false>
 __device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, double dt, int n_steps) {
   constexpr int NQ = nq_of<Topo>(), NV = nv_of<Topo>();
   const int inst = tid / L;
@@ -607,12 +750,13 @@ __device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, doub
   for (int k = 0; k < NV; ++k) v[k] = vS.ld(k, inst);
   bool ok = true;
   constexpr int PC = cached_bodies<Topo, R, ALLREV>();
-  const FastParams<Topo, R, PC> P(pick<R>(a.bp, a.bpf), Bp, inst);
-  LaunchConst<Topo, L, CONTACTS, PC == Topo::N> lc;
-  ok = load_launch_const(a, P, inst, sub, lc);
+  const FastParams<Topo, R, PC, LDS> P(pick<R>(a.bp, a.bpf), Bp, inst, 0, LDS ? (R*)GP_DYN_LDS : nullptr);
+  constexpr bool tau_reg = PC == Topo::N || FastParams<Topo, R, PC, LDS>::NT > 0;
+  LaunchConst<Topo, L, CONTACTS, tau_reg> lc;
+  ok = load_launch_const<Topo, R, L, CONTACTS>(a, P, inst, sub, lc);
   for (int s = 0; s < my_steps; ++s)
-    ok = fast_step<Topo, R, L, CONTACTS, ALLREV, true, PC>(a, P, inst, sub, q, v, dt, s + 1 == my_steps, lc,
-                                                          nullptr) && ok;
+    ok = fast_step<Topo, R, L, CONTACTS, ALLREV, true>(a, P, inst, sub, q, v, dt, s + 1 == my_steps, lc,
+                                                      nullptr) && ok;
   if (sub != 0) return;
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
@@ -629,7 +773,7 @@ __device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, doub
 
 template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
 __global__ void __launch_bounds__(256) fast_sie_kernel(BucketArgs a, double dt, int n_steps) {
-  fast_sie_body<Topo, R, L, CONTACTS, ALLREV>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps);
+  fast_sie_body<Topo, R, L, CONTACTS, ALLREV, true>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps);
 }
 
 // euler_step (integrators.rs:230-271): qo = q (+) h v, vo = v + h acc. In place
@@ -653,7 +797,7 @@ __device__ __forceinline__ void euler_stage(const double* q, const double* v, co
 // (a1 + 2 a2 + 2 a3 + a4) / 6 in that order. One copy of the dynamics serves
 // every stage (a stage loop that is not unrolled). vdot receives the last
 // step's first-stage accelerations, as the plain kernel leaves it.
-template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
+template <class Topo, class R, int L, bool CONTACTS, bool ALLREV, bool LDS = false>
 __device__ __forceinline__ void fast_rk_body(const BucketArgs& a, int tid, double dt, int n_steps, int integrator) {
   constexpr int NQ = nq_of<Topo>(), NV = nv_of<Topo>();
   const int inst = tid / L;
@@ -675,9 +819,10 @@ __device__ __forceinline__ void fast_rk_body(const BucketArgs& a, int tid, doubl
   for (int k = 0; k < NV; ++k) v[k] = vS.ld(k, inst);
   bool ok = true;
   constexpr int PC = cached_bodies<Topo, R, ALLREV>();
-  const FastParams<Topo, R, PC> P(pick<R>(a.bp, a.bpf), Bp, inst);
-  LaunchConst<Topo, L, CONTACTS, PC == Topo::N> lc;
-  ok = load_launch_const(a, P, inst, sub, lc);
+  const FastParams<Topo, R, PC, LDS> P(pick<R>(a.bp, a.bpf), Bp, inst, 0, LDS ? (R*)GP_DYN_LDS : nullptr);
+  constexpr bool tau_reg = PC == Topo::N || FastParams<Topo, R, PC, LDS>::NT > 0;
+  LaunchConst<Topo, L, CONTACTS, tau_reg> lc;
+  ok = load_launch_const<Topo, R, L, CONTACTS>(a, P, inst, sub, lc);
   const bool rk4 = integrator == GP_RUNGE_KUTTA_4;
   const int stages = rk4 ? 4 : 2;
   double qs[NQ], vs[NV], ak[NV], sum[NV];
@@ -688,7 +833,7 @@ __device__ __forceinline__ void fast_rk_body(const BucketArgs& a, int tid, doubl
     for (int k = 0; k < NV; ++k) vs[k] = v[k];
 #pragma unroll 1
     for (int st = 0; st < stages; ++st) {
-      ok = fast_step<Topo, R, L, CONTACTS, ALLREV, false, PC>(a, P, inst, sub, qs, vs, 0.0, false, lc, ak) && ok;
+      ok = fast_step<Topo, R, L, CONTACTS, ALLREV, false>(a, P, inst, sub, qs, vs, 0.0, false, lc, ak) && ok;
       if (st == 0) {
         if (s + 1 == my_steps && sub == 0) {
 #pragma unroll
@@ -723,7 +868,8 @@ __device__ __forceinline__ void fast_rk_body(const BucketArgs& a, int tid, doubl
 
 template <class Topo, class R, int L, bool CONTACTS, bool ALLREV>
 __global__ void __launch_bounds__(256) fast_rk_kernel(BucketArgs a, double dt, int n_steps, int integrator) {
-  fast_rk_body<Topo, R, L, CONTACTS, ALLREV>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps, integrator);
+  fast_rk_body<Topo, R, L, CONTACTS, ALLREV, true>(a, blockIdx.x * blockDim.x + threadIdx.x, dt, n_steps,
+                                                   integrator);
 }
 
 // ---------------------------------------------------------------- trees, one lane per leg
@@ -961,16 +1107,16 @@ __device__ __forceinline__ void grouped_dispatch(const BucketArgs& a, int topo, 
   switch (topo) {
 #define GP_CHAIN_CASE(K)                                                               \
   case K:                                                                              \
-    if (ar) fast_sie_body<TopoChain<K>, R, 1, false, true>(a, tid, dt, n);             \
-    else fast_sie_body<TopoChain<K>, R, 1, false, false>(a, tid, dt, n);               \
+    if (ar) fast_sie_body<TopoChain<K>, R, 1, false, true, true>(a, tid, dt, n);       \
+    else fast_sie_body<TopoChain<K>, R, 1, false, false, true>(a, tid, dt, n);         \
     break;
     GP_CHAIN_CASE(1) GP_CHAIN_CASE(2) GP_CHAIN_CASE(3) GP_CHAIN_CASE(4) GP_CHAIN_CASE(5) GP_CHAIN_CASE(6)
     GP_CHAIN_CASE(7) GP_CHAIN_CASE(8) GP_CHAIN_CASE(9) GP_CHAIN_CASE(10) GP_CHAIN_CASE(11) GP_CHAIN_CASE(12)
 #undef GP_CHAIN_CASE
 #define GP_WELDED_CASE(K)                                                              \
   case TOPO_WELDED + K:                                                                \
-    if (ar) fast_sie_body<TopoWelded<K + 1>, R, 1, false, true>(a, tid, dt, n);        \
-    else fast_sie_body<TopoWelded<K + 1>, R, 1, false, false>(a, tid, dt, n);          \
+    if (ar) fast_sie_body<TopoWelded<K + 1>, R, 1, false, true, true>(a, tid, dt, n);  \
+    else fast_sie_body<TopoWelded<K + 1>, R, 1, false, false, true>(a, tid, dt, n);    \
     break;
     GP_WELDED_CASE(1) GP_WELDED_CASE(2) GP_WELDED_CASE(3) GP_WELDED_CASE(4) GP_WELDED_CASE(5) GP_WELDED_CASE(6)
     GP_WELDED_CASE(7)
@@ -1014,11 +1160,46 @@ __global__ void __launch_bounds__(256) grouped_sie_kernel(const GroupTable* __re
 }
 
 
+// The largest tier row (bytes per lane) of any chain or welded-chain kernel
+// in this precision: a grouped launch reserves it for every lane, whichever
+// buckets the launch holds (its kernel runs at one wave per SIMD anyway).
+template <class R, bool AR, int... K>
+constexpr int max_tier_lane_bytes(std::integer_sequence<int, K...>) {
+  int m = 0;
+  ((m = m > lds_tier_bytes<TopoChain<K + 1>, R, AR>(1) ? m : lds_tier_bytes<TopoChain<K + 1>, R, AR>(1)), ...);
+  ((m = m > lds_tier_bytes<TopoWelded<K + 1>, R, AR>(1) ? m : lds_tier_bytes<TopoWelded<K + 1>, R, AR>(1)), ...);
+  return m;
+}
+template <class R> constexpr int grouped_tier_lane_bytes() {
+  constexpr int a = max_tier_lane_bytes<R, true>(std::make_integer_sequence<int, 12>{});
+  constexpr int b = max_tier_lane_bytes<R, false>(std::make_integer_sequence<int, 12>{});
+  return a > b ? a : b;
+}
+
+// Workgroup size that fits the tier's LDS in what the device allows a
+// workgroup (the requested size, halved while it does not fit; 64 lanes need
+// at most 40 KB).
+template <class Topo, class R, bool AR>
+static int tier_block(int block) {
+  static int max_lds = -1;
+  if (max_lds < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      v = 64 << 10;
+    max_lds = v;
+  }
+  while (block > 64 && lds_tier_bytes<Topo, R, AR>(block) > max_lds) block /= 2;
+  return block;
+}
+
 template <class Topo, class R, int L, bool C, bool AR>
 static int launch_t(const BucketArgs& a, double dt, int n_steps, int block, hipStream_t s) {
+  block = tier_block<Topo, R, AR>(block);
   long threads = (long)a.B * L;
   dim3 blk(block), grid((unsigned)((threads + block - 1) / block));
-  hipLaunchKernelGGL((fast_sie_kernel<Topo, R, L, C, AR>), grid, blk, 0, s, a, dt, n_steps);
+  const unsigned lds = (unsigned)lds_tier_bytes<Topo, R, AR>(block);
+  hipLaunchKernelGGL((fast_sie_kernel<Topo, R, L, C, AR>), grid, blk, lds, s, a, dt, n_steps);
   return (int)hipGetLastError();
 }
 
@@ -1039,9 +1220,11 @@ static int launch_tree(const BucketArgs& a, double dt, int n_steps, int block, h
 
 template <class Topo, class R, int L, bool C, bool AR>
 static int launch_rk_t(const BucketArgs& a, double dt, int n_steps, int integrator, int block, hipStream_t s) {
+  block = tier_block<Topo, R, AR>(block);
   long threads = (long)a.B * L;
   dim3 blk(block), grid((unsigned)((threads + block - 1) / block));
-  hipLaunchKernelGGL((fast_rk_kernel<Topo, R, L, C, AR>), grid, blk, 0, s, a, dt, n_steps, integrator);
+  const unsigned lds = (unsigned)lds_tier_bytes<Topo, R, AR>(block);
+  hipLaunchKernelGGL((fast_rk_kernel<Topo, R, L, C, AR>), grid, blk, lds, s, a, dt, n_steps, integrator);
   return (int)hipGetLastError();
 }
 

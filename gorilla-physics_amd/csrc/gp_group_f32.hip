@@ -4,7 +4,9 @@
 namespace gp {
 
 int launch_group_f32(const GroupTable* d_table, int n_blocks, double dt, int n_steps, int block, hipStream_t s) {
-  hipLaunchKernelGGL(grouped_sie_kernel<float>, dim3(n_blocks), dim3(block), 0, s, d_table, dt, n_steps);
+  // the chain kernels' LDS tier, at its largest row for every lane
+  const unsigned lds = (unsigned)(grouped_tier_lane_bytes<float>() * block);
+  hipLaunchKernelGGL(grouped_sie_kernel<float>, dim3(n_blocks), dim3(block), lds, s, d_table, dt, n_steps);
   return (int)hipGetLastError();
 }
 

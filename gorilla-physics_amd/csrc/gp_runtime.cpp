@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -127,6 +128,8 @@ int model_create(const gp_model_desc* d, gp_model** out) {
   if (!std::isfinite(d->gravity)) return fail(GP_ERR_INVALID, "gravity not finite");
   std::unique_ptr<gp_model> m(new (std::nothrow) gp_model());
   if (!m) return fail(GP_ERR_NOMEM, "out of host memory");
+  static std::atomic<uint64_t> next_serial{1};
+  m->serial = next_serial.fetch_add(1);
   const int n = d->n_bodies;
   m->n = n;
   m->g = d->gravity;

@@ -30,6 +30,7 @@ struct gp_model {
   std::vector<int> sphere_of;      // [n] sphere collider slot of the body or -1
   std::vector<double> shp;         // [ns][NSH]
   uint64_t sig = 0;
+  uint64_t serial = 0;  // unique per model created in the process (never reused)
   int fast_topo = -1;
   bool all_rev = true;  // no prismatic joint
 };

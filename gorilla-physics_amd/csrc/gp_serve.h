@@ -21,15 +21,21 @@ struct PoolPtrs {
   int* active;
 };
 
+// A registered model's parameters in device memory, uploaded once when the
+// model is first submitted (device pointers).
+struct ModelDev {
+  const int* kind;   // [nb]
+  const double* bp;  // [nb][NBF] (aligned-frame body parameters)
+  const double* cp;  // [nc][NCF]
+  const double* hp;  // [nh][NHF]
+};
+
 // K admitted requests, packed request after request (device pointers).
 struct AdmitBatch {
   int K;
   const int* lanes;   // [K]
-  const int* kind;    // [K][nb]
+  const int* model;   // [K] index into the table's models
   const int* steps;   // [K] horizon in steps
-  const double* bp;   // [K][nb][NBF] (aligned-frame body parameters)
-  const double* cp;   // [K][nc][NCF]
-  const double* hp;   // [K][nh][NHF]
   const double* q;    // [K][nq]
   const double* v;    // [K][nv]
   const double* tau;  // [K][nv]
@@ -48,6 +54,7 @@ struct GatherBatch {
 // launch: table entry e covers requests [start[e], start[e + 1]).
 struct AdmitTable {
   int n;
+  const ModelDev* models;  // the server's registered models
   int start[kMaxGroup + 1];
   PoolPtrs p[kMaxGroup];
   AdmitBatch x[kMaxGroup];

@@ -1,8 +1,10 @@
 // gp_serve_kernels.hip — slot admission and retirement for the serving loop.
 //
-// admit: the requests of every pool admitted in one round, packed on the host
-// one after another, are written into their slots (lanes) of the pools'
-// structure-of-arrays state and parameters by one launch. gather: the final
+// admit: the requests of every pool admitted in one round (their state packed
+// on the host one after another, their models' parameters from the device
+// copies made when the models were first submitted) are written into their
+// slots (lanes) of the pools' structure-of-arrays state and parameters by one
+// launch. gather: the final
 // state of every slot that finished in the round is packed into a compact
 // buffer for one device-to-host copy, by one launch.
 #include <hip/hip_runtime.h>
@@ -20,8 +22,9 @@ template <class T> __device__ __forceinline__ int entry_of(const T* t, int r) {
 }
 
 // Workgroup r writes request r into its slot: kind, body / contact /
-// half-space parameters (both precisions), q, v, tau, a zero vdot and fault,
-// and the horizon. Lane j handles elements j, j + 64, ... of the request.
+// half-space parameters (both precisions, from its model's device copy), q,
+// v, tau, a zero vdot and fault, and the horizon. Lane j handles elements
+// j, j + 64, ... of the request.
 __global__ void __launch_bounds__(64) admit_kernel(const AdmitTable* __restrict__ t) {
   const int r = blockIdx.x;
   const int e = entry_of(t, r);
@@ -30,15 +33,16 @@ __global__ void __launch_bounds__(64) admit_kernel(const AdmitTable* __restrict_
   const int k = r - t->start[e];
   const size_t Bp = p.Bp;
   const int l = x.lanes[k];
+  const ModelDev md = t->models[x.model[k]];
   const int nbp = p.nb * NBF, ncp = p.nc * NCF, nhp = p.nh * NHF;
   const int n_elem = p.nb + nbp + ncp + nhp + p.nq + 2 * p.nv;
   for (int j = threadIdx.x; j < n_elem; j += blockDim.x) {
     int m = j;
-    if (m < p.nb) { p.kind[m * Bp + l] = x.kind[k * p.nb + m]; continue; }
+    if (m < p.nb) { p.kind[m * Bp + l] = md.kind[m]; continue; }
     m -= p.nb;
     if (m < nbp) {  // packed [nb][NBF] -> SoA [NBF][nb]
       const int i = m / NBF, f = m % NBF;
-      const double val = x.bp[(size_t)k * nbp + m];
+      const double val = md.bp[m];
       p.bp[((size_t)f * p.nb + i) * Bp + l] = val;
       p.bpf[((size_t)f * p.nb + i) * Bp + l] = float(val);
       continue;
@@ -46,7 +50,7 @@ __global__ void __launch_bounds__(64) admit_kernel(const AdmitTable* __restrict_
     m -= nbp;
     if (m < ncp) {
       const int c = m / NCF, f = m % NCF;
-      const double val = x.cp[(size_t)k * ncp + m];
+      const double val = md.cp[m];
       p.cp[((size_t)f * p.nc + c) * Bp + l] = val;
       p.cpf[((size_t)f * p.nc + c) * Bp + l] = float(val);
       continue;
@@ -54,7 +58,7 @@ __global__ void __launch_bounds__(64) admit_kernel(const AdmitTable* __restrict_
     m -= ncp;
     if (m < nhp) {
       const int h = m / NHF, f = m % NHF;
-      const double val = x.hp[(size_t)k * nhp + m];
+      const double val = md.hp[m];
       p.hp[((size_t)f * p.nh + h) * Bp + l] = val;
       p.hpf[((size_t)f * p.nh + h) * Bp + l] = float(val);
       continue;

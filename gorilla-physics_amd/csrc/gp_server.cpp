@@ -44,6 +44,7 @@ namespace {
 
 struct Request {
   const gp_model* model = nullptr;
+  int model_index = -1;  // the model's entry in the server's device registry
   std::vector<double> q0, v0, tau;
   int64_t horizon = 0;
   double dt = 0, arrival = 0, done_t = 0;
@@ -112,6 +113,15 @@ struct gp_server {
   size_t d_stage_cap = 0;
   char* h_stage = nullptr;
   size_t h_stage_cap = 0;
+  // device registry of submitted models (their parameters uploaded once):
+  // serial -> index into models / d_models; storage bump-allocated in arenas
+  std::unordered_map<uint64_t, int> model_index;
+  std::vector<ModelDev> models;
+  ModelDev* d_models = nullptr;
+  size_t d_models_cap = 0;
+  bool models_dirty = false;
+  std::vector<char*> arenas;
+  size_t arena_used = 0, arena_cap = 0;
   double rounds = 0, env_steps = 0, busy_s = 0, admitted = 0, completed = 0, admit_s = 0, retire_s = 0;
   size_t mem_used = 0, budget = 0;
   int64_t running = 0;
@@ -138,6 +148,69 @@ int ensure_stage(gp_server* s, size_t bytes) {
     if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipHostMalloc(staging): %s", hipGetErrorString(e));
     s->h_stage_cap = cap;
   }
+  return GP_OK;
+}
+
+// Device memory for a registered model's parameters (256-byte aligned),
+// from 4 MiB arenas held until the server is destroyed.
+int arena_alloc(gp_server* s, size_t bytes, char** out) {
+  bytes = (bytes + 255) / 256 * 256;
+  if (s->arenas.empty() || s->arena_used + bytes > s->arena_cap) {
+    const size_t cap = std::max(bytes, (size_t)4 << 20);
+    char* a = nullptr;
+    hipError_t e = hipMalloc((void**)&a, cap);
+    if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(model registry): %s", hipGetErrorString(e));
+    s->arenas.push_back(a);
+    s->arena_used = 0;
+    s->arena_cap = cap;
+  }
+  *out = s->arenas.back() + s->arena_used;
+  s->arena_used += bytes;
+  return GP_OK;
+}
+
+// The model's index in the device registry, uploading its parameters (joint
+// kinds, body, contact and half-space values) the first time it is seen.
+int register_model(gp_server* s, const gp_model* m, int* out) {
+  auto it = s->model_index.find(m->serial);
+  if (it != s->model_index.end()) { *out = it->second; return GP_OK; }
+  const size_t kb = sizeof(int) * (size_t)m->n, bb = sizeof(double) * m->bp.size(),
+               cb = sizeof(double) * m->cp.size(), hb = sizeof(double) * m->hp.size();
+  const size_t o_b = (kb + 255) / 256 * 256, o_c = o_b + (bb + 255) / 256 * 256, o_h = o_c + (cb + 255) / 256 * 256;
+  char* d = nullptr;
+  int st = arena_alloc(s, o_h + hb + 8, &d);
+  if (st) return st;
+  std::vector<char> h(o_h + hb + 8, 0);
+  std::memcpy(h.data(), m->kind.data(), kb);
+  if (bb) std::memcpy(h.data() + o_b, m->bp.data(), bb);
+  if (cb) std::memcpy(h.data() + o_c, m->cp.data(), cb);
+  if (hb) std::memcpy(h.data() + o_h, m->hp.data(), hb);
+  hipError_t e = hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "model upload: %s", hipGetErrorString(e));
+  s->models.push_back(ModelDev{(const int*)d, (const double*)(d + o_b), (const double*)(d + o_c),
+                               (const double*)(d + o_h)});
+  const int idx = (int)s->models.size() - 1;
+  s->model_index.emplace(m->serial, idx);
+  s->models_dirty = true;
+  *out = idx;
+  return GP_OK;
+}
+
+// The registry's table of device pointers, on the device.
+int sync_models(gp_server* s) {
+  if (!s->models_dirty) return GP_OK;
+  if (s->models.size() > s->d_models_cap) {
+    const size_t cap = std::max(s->models.size(), 2 * s->d_models_cap + 64);
+    if (s->d_models) (void)hipFree(s->d_models);
+    s->d_models = nullptr;
+    s->d_models_cap = 0;
+    hipError_t e = hipMalloc((void**)&s->d_models, cap * sizeof(ModelDev));
+    if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(model table): %s", hipGetErrorString(e));
+    s->d_models_cap = cap;
+  }
+  hipError_t e = hipMemcpy(s->d_models, s->models.data(), s->models.size() * sizeof(ModelDev), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(GP_ERR_DEVICE, "model table upload: %s", hipGetErrorString(e));
+  s->models_dirty = false;
   return GP_OK;
 }
 
@@ -259,8 +332,10 @@ int rebuild_group(gp_server* s) {
 // copy the whole in one transfer and write them into their slots with one
 // launch.
 int admit_pools(gp_server* s, std::map<int, std::vector<int64_t>>& adm) {
-  // layout: the table, then per pool: lanes, kind, steps (ints) then bp, cp, hp, q, v, tau (doubles)
-  struct Off { size_t lanes, kind, steps, bp, cp, hp, q, v, tau; int K; };
+  int st = sync_models(s);
+  if (st) return st;
+  // layout: the table, then per pool: lanes, model, steps (ints) then q, v, tau (doubles)
+  struct Off { size_t lanes, model, steps, q, v, tau; int K; };
   std::map<int, Off> offs;
   size_t at = round_up<double>(sizeof(AdmitTable));
   for (auto& kv : adm) {
@@ -269,43 +344,36 @@ int admit_pools(gp_server* s, std::map<int, std::vector<int64_t>>& adm) {
     Off o{};
     o.K = (int)K;
     o.lanes = at; at += round_up<double>(K * sizeof(int));
-    o.kind = at; at += round_up<double>(K * b.nb * sizeof(int));
+    o.model = at; at += round_up<double>(K * sizeof(int));
     o.steps = at; at += round_up<double>(K * sizeof(int));
-    o.bp = at; at += K * b.nb * NBF * sizeof(double);
-    o.cp = at; at += K * b.nc * NCF * sizeof(double);
-    o.hp = at; at += K * b.nh * NHF * sizeof(double);
     o.q = at; at += K * std::max(1, b.nq) * sizeof(double);
     o.v = at; at += K * std::max(1, b.nv) * sizeof(double);
     o.tau = at; at += K * std::max(1, b.nv) * sizeof(double);
     offs[kv.first] = o;
   }
-  int st = ensure_stage(s, at);
+  st = ensure_stage(s, at);
   if (st) return st;
   char* h = s->h_stage;
   char* d = s->d_stage;
   AdmitTable* tab = (AdmitTable*)h;
   std::memset(tab, 0, sizeof *tab);
+  tab->models = s->d_models;
   int total = 0;
   for (auto& kv : adm) {
     const Off& o = offs[kv.first];
     const Bucket& b = s->pools[kv.first]->b;
     for (int k = 0; k < o.K; ++k) {
       const Request& r = s->req[kv.second[k]];
-      const gp_model* m = r.model;
       ((int*)(h + o.lanes))[k] = r.slot;
-      for (int i = 0; i < b.nb; ++i) ((int*)(h + o.kind))[k * b.nb + i] = m->kind[i];
+      ((int*)(h + o.model))[k] = r.model_index;
       ((int*)(h + o.steps))[k] = (int)r.horizon;
-      std::memcpy(h + o.bp + (size_t)k * b.nb * NBF * 8, m->bp.data(), (size_t)b.nb * NBF * 8);
-      if (b.nc) std::memcpy(h + o.cp + (size_t)k * b.nc * NCF * 8, m->cp.data(), (size_t)b.nc * NCF * 8);
-      if (b.nh) std::memcpy(h + o.hp + (size_t)k * b.nh * NHF * 8, m->hp.data(), (size_t)b.nh * NHF * 8);
       std::memcpy(h + o.q + (size_t)k * b.nq * 8, r.q0.data(), (size_t)b.nq * 8);
       std::memcpy(h + o.v + (size_t)k * b.nv * 8, r.v0.data(), (size_t)b.nv * 8);
       std::memcpy(h + o.tau + (size_t)k * b.nv * 8, r.tau.data(), (size_t)b.nv * 8);
     }
     tab->start[tab->n] = total;
     tab->p[tab->n] = pool_ptrs(*s->pools[kv.first]);
-    tab->x[tab->n] = AdmitBatch{o.K, (const int*)(d + o.lanes), (const int*)(d + o.kind), (const int*)(d + o.steps),
-                                (const double*)(d + o.bp), (const double*)(d + o.cp), (const double*)(d + o.hp),
+    tab->x[tab->n] = AdmitBatch{o.K, (const int*)(d + o.lanes), (const int*)(d + o.model), (const int*)(d + o.steps),
                                 (const double*)(d + o.q), (const double*)(d + o.v), (const double*)(d + o.tau)};
     total += o.K;
     ++tab->n;
@@ -543,6 +611,8 @@ void gp_server_destroy(gp_server* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   s->pools.clear();
   if (s->d_group) (void)hipFree(s->d_group);
+  if (s->d_models) (void)hipFree(s->d_models);
+  for (char* a : s->arenas) (void)hipFree(a);
   if (s->d_stage) (void)hipFree(s->d_stage);
   if (s->h_stage) (void)hipHostFree(s->h_stage);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -564,6 +634,8 @@ int gp_server_submit(gp_server* s, const gp_model* m, const double* q0, const do
   try {
     Request r;
     r.model = m;
+    int st = register_model(s, m, &r.model_index);
+    if (st) return st;
     r.q0.assign(q0, q0 + m->nq);
     r.v0.assign(v0, v0 + m->nv);
     r.tau.assign(m->nv, 0.0);

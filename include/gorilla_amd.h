@@ -380,7 +380,10 @@ int gp_server_create(const gp_server_opts* opts, gp_server** out);
 void gp_server_destroy(gp_server* server);
 /* seconds since the server was created */
 int gp_server_clock(const gp_server* server, double* now_s);
-/* Queue a request that arrives at arrival_s on the server clock (< 0: now). */
+/* Queue a request that arrives at arrival_s on the server clock (< 0: now).
+ * The first request of a model uploads the model's parameters to the device
+ * once (a registry the server keeps until it is destroyed); admissions then
+ * copy only each request's state, torques and horizon. */
 int gp_server_submit(gp_server* server, const gp_model* model, const double* q0, const double* v0,
                      const double* tau, int64_t horizon_steps, double dt, double arrival_s,
                      int64_t* request_id);
@@ -396,9 +399,10 @@ int gp_server_result(gp_server* server, int64_t request_id, double* q, double* v
 /* 10 values: [rounds, env_steps, busy_seconds, admitted, completed, pools,
  *  pool_bytes, budget_bytes, admit_seconds, retire_seconds]; busy_seconds is
  *  the host time from each round's launch to its results, admit_seconds the
- *  host time spent admitting (packing, one transfer, one launch per round),
- *  retire_seconds the time spent packing and copying back finished slots
- *  (inside busy_seconds). */
+ *  host time spent admitting (packing, one transfer and one launch per 32
+ *  pools per round, its wait), retire_seconds the time spent packing and
+ *  copying back finished slots after the round's steps ended (inside
+ *  busy_seconds). */
 int gp_server_stats(const gp_server* server, double* stats);
 
 #ifdef __cplusplus

@@ -1,10 +1,11 @@
 // Host stand-in for <hip/hip_runtime.h>, used only by the CPU build of the
-// library's runtime-topology kernels under AddressSanitizer / UBSan
+// library's runtime and kernels under AddressSanitizer / UBSan
 // (tests/test_host_sanitized_kernels.py). Device memory is host memory
 // allocated with its exact size (so the sanitizer sees every access past an
 // array), a kernel launch runs every (workgroup, lane) in turn on the calling
 // thread, and streams, events and graphs are no-ops. Kernels that exchange
-// values between lanes (shuffles, LDS, barriers) are not built this way.
+// values between lanes (shuffles, shared LDS, barriers) are not built this
+// way; lane-private LDS columns are.
 #pragma once
 
 #include <algorithm>
@@ -156,16 +157,31 @@ template <class T> inline T gp_host_no_shuffle() {
 template <class T> inline T __shfl_xor(T, int, int = 64) { return gp_host_no_shuffle<T>(); }
 template <class T> inline T __shfl(T, int, int = 64) { return gp_host_no_shuffle<T>(); }
 
+// Dynamic LDS of the running launch: one guarded allocation of the launch's
+// size, shared by its workgroups in turn (the kernels that use it keep
+// lane-private columns, so one workgroup's values never reach another's lanes
+// before being written).
+inline thread_local unsigned char* gp_host_dyn_lds = nullptr;
+#define GP_DYN_LDS gp_host_dyn_lds
+enum hipDeviceAttribute_t { hipDeviceAttributeMaxSharedMemoryPerBlock };
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 160 << 10; return hipSuccess; }
+
 // every (workgroup, lane) of the grid in turn
 #define hipLaunchKernelGGL(kernel, grid, block, shmem, stream, ...)          \
   do {                                                                       \
     const dim3 gp_g_ = dim3(grid), gp_b_ = dim3(block);                      \
     gridDim.x = gp_g_.x;                                                     \
     blockDim.x = gp_b_.x;                                                    \
+    const size_t gp_sh_ = (size_t)(shmem);                                   \
+    void* gp_lds_ = nullptr;                                                 \
+    if (gp_sh_) (void)hipMalloc(&gp_lds_, gp_sh_);                           \
+    gp_host_dyn_lds = (unsigned char*)gp_lds_;                               \
     for (unsigned gp_bx = 0; gp_bx < gp_g_.x; ++gp_bx)                       \
       for (unsigned gp_tx = 0; gp_tx < gp_b_.x; ++gp_tx) {                   \
         blockIdx.x = gp_bx;                                                  \
         threadIdx.x = gp_tx;                                                 \
         kernel(__VA_ARGS__);                                                 \
       }                                                                      \
+    gp_host_dyn_lds = nullptr;                                               \
+    if (gp_lds_) (void)hipFree(gp_lds_);                                     \
   } while (0)
