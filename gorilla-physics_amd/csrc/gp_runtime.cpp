@@ -463,6 +463,26 @@ BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau) {
   return a;
 }
 
+// Time of one step of one wave of a bucket's compiled kernel in microseconds,
+// by topology and precision (measured on MI355X with 16,384 instances per
+// topology, before the FP64 chains' LDS tier; entries for 5, 7, 9 and 11
+// joints interpolated). Per-lane revolute/prismatic kinds add about a fifth.
+// Only the ratios matter: they order a grouped launch's buckets and set the
+// serving loop's steps per pool.
+double bucket_step_cost(const Bucket& b, int precision, int lanes_per_tree) {
+  static const double chain_mixed[12] = {0.25, 1.07, 2.13, 3.08, 4.2, 5.27, 6.4, 7.6, 11.5, 15.5, 17.9, 20.2};
+  static const double chain_f64[12] = {0.5, 1.35, 2.77, 4.16, 7.3, 10.5, 14.2, 17.9, 33.4, 48.9, 61.7, 74.5};
+  const bool mixed = precision == GP_PREC_MIXED;
+  const int t = b.fast_topo;
+  if (t >= 1 && t <= 12) return (mixed ? chain_mixed : chain_f64)[t - 1] * (b.all_rev ? 1.0 : 1.2);
+  if (t > TOPO_WELDED && t <= TOPO_WELDED + kMaxWeldedJoints)  // a welded base costs nothing per step
+    return (mixed ? chain_mixed : chain_f64)[t - TOPO_WELDED - 1] * (b.all_rev ? 1.0 : 1.2);
+  if (t == TOPO_QUAD && lanes_per_tree == 1) return mixed ? 29.4 : 116.0;
+  if (t == TOPO_QUAD) return mixed ? 7.0 : 14.3;   // one lane per leg
+  if (t == TOPO_FREE) return mixed ? 1.4 : 1.7;    // four lanes per body, 8 contact points
+  return mixed ? 20.0 : 75.0;
+}
+
 }  // namespace gp
 
 using namespace gp;
@@ -500,8 +520,17 @@ static int ensure_group(gp_batch* bt, int integrator) {
   if (bt->group_valid && bt->group_block == block) return GP_OK;
   GroupTable t{};
   int blocks = 0;
-  for (const Bucket& b : bt->buckets) {
-    if (!bucket_is_fast(o, b, integrator)) continue;
+  // costliest steps first: workgroups start in index order, so the longest
+  // ones do not form the launch's tail
+  std::vector<const Bucket*> order;
+  for (const Bucket& b : bt->buckets)
+    if (bucket_is_fast(o, b, integrator)) order.push_back(&b);
+  const int tree = integrator == GP_SEMI_IMPLICIT_EULER ? o.lanes_per_tree : 1;
+  std::stable_sort(order.begin(), order.end(), [&](const Bucket* x, const Bucket* y) {
+    return bucket_step_cost(*x, o.precision, tree) > bucket_step_cost(*y, o.precision, tree);
+  });
+  for (const Bucket* bb : order) {
+    const Bucket& b = *bb;
     const int lanes = bucket_lanes(o, b, integrator);
     t.args[t.n] = bucket_args(b, lanes, nullptr);
     t.topo[t.n] = b.fast_topo;
@@ -519,6 +548,15 @@ static int ensure_group(gp_batch* bt, int integrator) {
   bt->group_block = block;
   bt->group_valid = true;
   return GP_OK;
+}
+
+// Contact candidates of a bucket that the single-body velocity-stepping
+// kernel can take (one floating body, at most 16 point / sphere x half-space
+// pairs), else -1.
+static int vs_body_candidate_count(const Bucket& b) {
+  if (b.nb != 1 || b.dofcls.empty() || b.dofcls[0] != 6) return -1;
+  const int P = b.nc * b.nh + (b.ns > 0 ? b.nh : 0);
+  return P <= 16 ? P : -1;
 }
 
 // Every bucket's launches for n_steps steps. scratch_override: where the
@@ -548,14 +586,28 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
     if (is_velocity_stepping(integrator)) {
       if (o.precision == GP_PREC_MIXED && o.kernel == GP_KERNEL_PLAIN)
         return fail(GP_ERR_UNSUPPORTED, "velocity stepping computes in FP64 only");
+      const int ccd = integrator == GP_CCD_VELOCITY_STEPPING ? 1 : 0;
+      const int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
+      const int cand = vs_body_candidate_count(b);
+      if (o.kernel != GP_KERNEL_PLAIN && cand >= 0) {
+        // a lone floating body: the register-resident kernel, several lanes per instance
+        const int lanes = o.lanes_per_mech == 1 ? 1 : 8;
+        BucketArgs a = bucket_args(b, lanes, nullptr);
+        for (int done = 0; done < n_steps; done += spl) {
+          const int n = std::min(spl, n_steps - done);
+          int e = launch_vs_body(a, dt, ccd, n, lanes, cand, s);
+          if (e) return fail(GP_ERR_DEVICE, "velocity-stepping launch: %s", hipGetErrorString((hipError_t)e));
+          ++*launches;
+        }
+        continue;
+      }
       int st = scratch_override ? GP_OK : bucket_ensure_scratch(b);
       if (st) return st;
       BucketArgs a = bucket_args(b, 1, nullptr);
       if (scratch_override) a.scratch = scratch_override;
-      const int spl = o.steps_per_launch > 0 ? o.steps_per_launch : 100;
       for (int done = 0; done < n_steps; done += spl) {
         const int n = std::min(spl, n_steps - done);
-        int e = launch_vs_step(a, dt, integrator == GP_CCD_VELOCITY_STEPPING ? 1 : 0, n, s);
+        int e = launch_vs_step(a, dt, ccd, n, s);
         if (e) return fail(GP_ERR_DEVICE, "velocity-stepping launch: %s", hipGetErrorString((hipError_t)e));
         ++*launches;
       }
@@ -989,6 +1041,15 @@ static int chunk_graph(gp_batch* bt, double dt, int integrator, int steps, hipSt
   return GP_OK;
 }
 
+// The chunks of a graph-launched rollout of n_steps: steps per graph launch,
+// full chunks of steps_per_launch then the rest.
+static std::vector<int> chunk_plan(gp_batch* bt, int n_steps) {
+  const int spl = chunk_steps(bt);
+  std::vector<int> plan;
+  for (int done = 0; done < n_steps; done += spl) plan.push_back(std::min(spl, n_steps - done));
+  return plan;
+}
+
 // Everything a rollout of n_steps needs before its first launch: the grouped
 // launch's table and, with graphs on, the chunk graphs.
 static int step_prepare(gp_batch* bt, double dt, int integrator, int n_steps, hipStream_t s) {
@@ -997,14 +1058,12 @@ static int step_prepare(gp_batch* bt, double dt, int integrator, int n_steps, hi
     if (st) return st;
   }
   if (!bt->opts.use_graph || n_steps <= 0) return GP_OK;
-  const int spl = chunk_steps(bt);
   hipGraphExec_t ex;
   int64_t l = 0;
-  if (n_steps >= spl) {
-    int st = chunk_graph(bt, dt, integrator, spl, s, &ex, &l);
+  for (int c : chunk_plan(bt, n_steps)) {
+    int st = chunk_graph(bt, dt, integrator, c, s, &ex, &l);
     if (st) return st;
   }
-  if (n_steps % spl) return chunk_graph(bt, dt, integrator, n_steps % spl, s, &ex, &l);
   return GP_OK;
 }
 
@@ -1020,21 +1079,13 @@ int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* 
   hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
   if ((st = step_prepare(bt, dt, integrator, n_steps, s))) return st;
   int64_t launches = 0;
-  // the timing events bracket the launches only (any capture is done above)
+  // the timing events bracket the launches only (any preparation is done above)
   HIP_TRY(hipEventRecord(bt->ev0, s));
   if (bt->opts.use_graph && n_steps > 0) {
-    const int spl = chunk_steps(bt);
-    hipGraphExec_t ex;
-    int64_t l = 0;
-    if (n_steps >= spl) {
-      if ((st = chunk_graph(bt, dt, integrator, spl, s, &ex, &l))) return st;
-      for (int done = 0; done + spl <= n_steps; done += spl) {
-        HIP_TRY(hipGraphLaunch(ex, s));
-        launches += l;
-      }
-    }
-    if (n_steps % spl) {
-      if ((st = chunk_graph(bt, dt, integrator, n_steps % spl, s, &ex, &l))) return st;
+    for (int c : chunk_plan(bt, n_steps)) {
+      hipGraphExec_t ex;
+      int64_t l = 0;
+      if ((st = chunk_graph(bt, dt, integrator, c, s, &ex, &l))) return st;
       HIP_TRY(hipGraphLaunch(ex, s));
       launches += l;
     }

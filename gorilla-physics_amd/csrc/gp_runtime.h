@@ -106,6 +106,8 @@ BucketArgs bucket_args(const Bucket& b, int lanes, const double* tau);
 // per-bucket array ([rows][Bp]) with 32-bit byte offsets, so the largest one
 // stays below 2^31 bytes.
 int64_t soa_max_lanes(const gp_model& m);
+// Relative time of one step of one wave of a bucket's compiled kernel.
+double bucket_step_cost(const Bucket& b, int precision, int lanes_per_tree);
 
 }  // namespace gp
 
@@ -119,6 +121,7 @@ struct gp_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timing_valid = false;
   int64_t last_launches = 0;
+  // chunk graphs by (dt, integrator, steps)
   std::map<std::tuple<double, int, int>, hipGraphExec_t> graphs;
   std::map<std::tuple<double, int, int>, int64_t> graph_launches;
   // grouped launch over the FAST buckets (built lazily, rebuilt on set_opts)

@@ -59,7 +59,7 @@ struct Pool {
   uint64_t sig = 0;
   double dt = 0;
   bool all_rev = false;
-  double cost = 1;  // relative time of one step of one wave (step_cost)
+  double cost = 1;  // relative time of one step of one wave (bucket_step_cost)
   int chunk = 0;    // steps per round
   Bucket b;
   std::vector<int64_t> slot_req;  // request id per slot, -1 = free
@@ -69,24 +69,6 @@ struct Pool {
 };
 
 constexpr int kMaxChunkScale = 32;
-
-// Time of one step of one wave of a pool's kernel in microseconds, by
-// compiled topology and precision (measured on MI355X with 16,384 instances
-// per topology; entries for 5, 7, 9 and 11 joints interpolated). Per-lane
-// revolute/prismatic kinds add about a fifth. Only the ratios matter.
-double step_cost(const Bucket& b, int precision, int lanes_per_tree) {
-  static const double chain_mixed[12] = {0.25, 1.07, 2.13, 3.08, 4.2, 5.27, 6.4, 7.6, 11.5, 15.5, 17.9, 20.2};
-  static const double chain_f64[12] = {0.5, 1.35, 2.77, 4.16, 7.3, 10.5, 14.2, 17.9, 33.4, 48.9, 61.7, 74.5};
-  const bool mixed = precision == GP_PREC_MIXED;
-  const int t = b.fast_topo;
-  if (t >= 1 && t <= 12) return (mixed ? chain_mixed : chain_f64)[t - 1] * (b.all_rev ? 1.0 : 1.2);
-  if (t > TOPO_WELDED && t <= TOPO_WELDED + kMaxWeldedJoints)  // a welded base costs nothing per step
-    return (mixed ? chain_mixed : chain_f64)[t - TOPO_WELDED - 1] * (b.all_rev ? 1.0 : 1.2);
-  if (t == TOPO_QUAD && lanes_per_tree == 1) return mixed ? 29.4 : 116.0;
-  if (t == TOPO_QUAD) return mixed ? 7.0 : 14.3;   // one lane per leg
-  if (t == TOPO_FREE) return mixed ? 1.4 : 1.7;    // four lanes per body, 8 contact points
-  return mixed ? 20.0 : 75.0;
-}
 
 template <class T>
 size_t round_up(size_t x) {
@@ -270,7 +252,7 @@ int find_pool(gp_server* s, const Request& r, int* out) {
   std::vector<const gp_model*> lanes(cap, m);
   int st = bucket_build(p->b, lanes, false);
   if (st) return st;
-  p->cost = step_cost(p->b, s->o.precision, s->o.lanes_per_tree);
+  p->cost = bucket_step_cost(p->b, s->o.precision, s->o.lanes_per_tree);
   hipError_t e = hipMalloc((void**)&p->b.d_active, sizeof(int) * Bp);
   if (e != hipSuccess) return fail(GP_ERR_NOMEM, "hipMalloc(active): %s", hipGetErrorString(e));
   e = hipMemset(p->b.d_active, 0, sizeof(int) * Bp);

@@ -82,9 +82,13 @@ enum {
 
 /* ---- kernel family --------------------------------------------------------- */
 enum {
-  GP_KERNEL_AUTO = 0,  /* FAST when the bucket topology has a compiled kernel */
+  GP_KERNEL_AUTO = 0,  /* FAST when the bucket topology has a compiled kernel;
+                          velocity stepping: the register-resident kernel for
+                          a lone floating body with at most 8 contact
+                          candidates, the runtime-topology one otherwise     */
   GP_KERNEL_PLAIN = 1, /* runtime-topology FP64 kernel, per-body
-                          intermediates in device scratch                     */
+                          intermediates in device scratch (also for velocity
+                          stepping)                                          */
   GP_KERNEL_FAST = 2   /* compile-time topology, register resident, several
                           steps per launch                                     */
 };
@@ -172,7 +176,10 @@ typedef struct gp_batch_opts {
                                 (0 = 100)                                     */
   int32_t block_size;        /* threads per workgroup (0 = 256)               */
   int32_t lanes_per_mech;    /* FAST free bodies with contacts: lanes sharing
-                                one instance's contact points, 1/2/4/8 (0 = 4) */
+                                one instance's contact points, 1/2/4/8 (0 = 4);
+                                velocity stepping of a lone floating body with
+                                at most 8 contact candidates: 1 = one lane
+                                per body, else 8 lanes per body             */
   int32_t use_graph;         /* capture multi-launch rollouts in a hipGraph   */
   int32_t separate_launches; /* 1: one launch per topology bucket instead of
                                 one grouped launch over all FAST buckets      */

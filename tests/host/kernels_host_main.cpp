@@ -6,7 +6,8 @@
 // array is a host allocation of its exact size between poisoned guards, every
 // lane runs in turn. Reads a batch written by the host-build tests, runs the
 // entry points over it through the C ABI, prints a checksum and, given an
-// output file, writes the final q, v and vdot there.
+// output file, writes the final q, v and vdot (doubles) and the last step's
+// collision-loop rounds per instance (int32) there.
 //
 //   kernels_host <batch file> [out file] [kernel] [precision]
 //
@@ -123,6 +124,9 @@ int main(int argc, char** argv) {
     std::fwrite(q.data(), sizeof(double), nq, o);
     std::fwrite(v.data(), sizeof(double), nv, o);
     std::fwrite(vdot.data(), sizeof(double), nv, o);
+    std::vector<int32_t> rr(ni);
+    need(gp_batch_collision_rounds(b, rr.data()) == GP_OK, "rounds");
+    std::fwrite(rr.data(), sizeof(int32_t), ni, o);
     std::fclose(o);
   }
   need(gp_batch_energy(b, ke.data(), pe.data()) == GP_OK, "energy");

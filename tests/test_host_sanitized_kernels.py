@@ -22,7 +22,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "gorilla-physics_amd", "csrc")
-SOURCES = ["gp_runtime.cpp", "gp_plain.hip", "gp_vstep.hip", "gp_energy.hip", "gp_terms.hip", "gp_xfer.hip"]
+SOURCES = ["gp_runtime.cpp", "gp_plain.hip", "gp_vstep.hip", "gp_vstep_body.hip", "gp_energy.hip", "gp_terms.hip",
+           "gp_xfer.hip"]
 
 
 FAST_SOURCES = ["gp_fast.hip", "gp_fast_f64.hip", "gp_fast_f32.hip", "gp_fast_rk_f64.hip", "gp_fast_rk_f32.hip"]
@@ -104,6 +105,20 @@ def test_thrown_cubes_velocity_stepping_in_bounds(gp, host_binary, tmp_path, int
     assert "faults 0" in out
 
 
+def test_thrown_cubes_single_body_kernel_in_bounds(gp, host_binary, tmp_path):
+    """The benchmark's thrown cubes through the single-body velocity-stepping
+    kernel (one lane per cube), CCD, the second's 120 steps in one launch."""
+    from gorilla_physics_amd import workloads
+    w = workloads.thrown_cubes(n=64)
+    out = tmp_path / "out.bin"
+    _run(host_binary, tmp_path, out=out, kernel=0, precision=0, models=w["models"], counts=w["counts"], q=w["q"],
+         v=w["v"], tau=w["tau"], dt=w["dt"], integrator=4, steps=120, spl=120)
+    q1, v1, _, rounds = _read_out(out, w["q"].size, w["v"].size)
+    assert np.isfinite(q1).all() and np.isfinite(v1).all() and (rounds >= 1).all()
+    z = q1.reshape(-1, 7)[:, 6]
+    assert (z > 0.0).all() and (z < 0.2).all()  # resting on the plane, not through it
+
+
 def test_velocity_stepping_models_in_bounds(gp, host_binary, tmp_path):
     """Spheres, cubes and quadrupeds on a tilted plane (the GPU velocity-
     stepping test's mix: sphere colliders, several contacts per body, trees),
@@ -157,6 +172,81 @@ def test_plain_kernel_and_terms_in_bounds(gp, host_binary, tmp_path):
          tau=rng.normal(0.0, 0.2, v.size), dt=1e-3, integrator=2, steps=50, spl=25)
 
 
+def _read_out(path, nq, nv):
+    """q, v, vdot and the collision-loop rounds the host driver wrote."""
+    raw = open(path, "rb").read()
+    d = np.frombuffer(raw[:8 * (nq + 2 * nv)], dtype=np.float64)
+    return d[:nq], d[nq:nq + nv], d[nq + nv:], np.frombuffer(raw[8 * (nq + 2 * nv):], dtype=np.int32)
+
+
+def _vs_mix(gp, rng, n=48):
+    """Cubes and spheres near a tilted ground (touching, sliding, in flight)
+    and quadrupeds standing on it."""
+    import math
+    from gorilla_physics_amd import builders
+    normal = np.array([math.sin(0.1), 0.0, math.cos(0.1)])
+    states = [builders.build_cube(1.0, 1.0), builders.build_sphere(1.0, 0.5), builders.build_quadruped()]
+    for st in states:
+        st.add_halfspace(gp.HalfSpace.new(normal, 0.0))
+    dc, ds, dq = (st.model_desc() for st in states)
+    descs, q, v = [], [], []
+    for k in range(n):
+        axis = rng.normal(size=3)
+        rot = gp.UnitQuaternion.from_axis_angle(axis / np.linalg.norm(axis), rng.uniform(0, 2 * math.pi))
+        if k % 3 == 0:
+            descs.append(dc)
+            q += [*rot.coords(), *rng.uniform(-1, 1, 2), rng.uniform(0.3, 1.0)]
+            v += [*rng.normal(size=3), *rng.normal(size=3)]
+        elif k % 3 == 1:
+            descs.append(ds)
+            q += [*rot.coords(), *rng.uniform(-1, 1, 2), rng.uniform(0.45, 0.55)]
+            v += [*rng.normal(size=3), *rng.normal(size=3)]
+        else:
+            descs.append(dq)
+            qq = np.zeros(dq.nq)
+            qq[3], qq[6] = 1.0, rng.uniform(0.9, 1.3)
+            qq[7:] = rng.normal(size=dq.nq - 7) * 0.2
+            q += list(qq)
+            v += list(rng.normal(size=dq.nv) * 0.3)
+    return descs, np.array(q), np.array(v)
+
+
+@pytest.mark.parametrize("integrator", [3, 4])
+def test_single_body_velocity_stepping_matches_the_oracle(gp, oracle, host_binary, tmp_path, integrator):
+    """The register-resident single-body velocity-stepping kernel (one lane
+    per body here: its cross-lane sums are identities), under the
+    sanitizers, one step from states along an oracle trajectory of cubes and
+    spheres on a tilted plane (quadrupeds beside them run the runtime-topology
+    kernel): the same collision-loop rounds as the oracle, q and v within the
+    1e-5 per-step tolerance of the GPU velocity-stepping tests."""
+    rng = np.random.default_rng(4)
+    descs, q, v = _vs_mix(gp, rng)
+    qo = np.cumsum([0] + [d.nq for d in descs])
+    vo = np.cumsum([0] + [d.nv for d in descs])
+    dt, worst = 1e-2, 0.0
+    for k in range(4):
+        out = tmp_path / f"out{k}.bin"
+        _run(host_binary, tmp_path, out=out, kernel=0, precision=0, models=descs, counts=[1] * len(descs), q=q, v=v,
+             tau=np.zeros(v.size), dt=dt, integrator=integrator, steps=1, spl=1)
+        q1, v1, _, rounds = _read_out(out, q.size, v.size)
+        qn, vn = q.copy(), v.copy()
+        for i, d in enumerate(descs):
+            qs, vs = slice(qo[i], qo[i + 1]), slice(vo[i], vo[i + 1])
+            rq, rv, ro = oracle.vs_step(d, q[qs], v[vs], np.zeros(d.nv), dt, integrator == 4)
+            assert rounds[i] == ro, (k, i, rounds[i], ro)
+            e = max(np.abs(q1[qs] - rq).max() / max(1, np.abs(rq).max()),
+                    np.abs(v1[vs] - rv).max() / max(1, np.abs(rv).max()))
+            worst = max(worst, e)
+            assert e < 1e-5, (k, i, e)
+            qn[qs], vn[vs] = rq, rv
+        for _ in range(4):  # the next states along the oracle's trajectory
+            for i, d in enumerate(descs):
+                qs, vs = slice(qo[i], qo[i + 1]), slice(vo[i], vo[i + 1])
+                qn[qs], vn[vs], _ = oracle.vs_step(d, qn[qs], vn[vs], np.zeros(d.nv), dt, integrator == 4)
+        q, v = qn, vn
+    print(f"single-body velocity stepping, integrator {integrator}: worst per-step difference {worst:.3g}")
+
+
 def _compiled_workload(gp):
     """Chains (arms, acrobots, the mixed workload's chains of 1-12 revolute
     or sprung prismatic joints), quadrupeds on one lane, free cubes with
@@ -201,8 +291,7 @@ def test_compiled_kernels_in_bounds_and_match_the_oracle(gp, oracle, host_binary
     out = tmp_path / "out.bin"
     _run(host_binary_fast, tmp_path, out=out, kernel=2, precision=precision, models=models, counts=counts, q=q, v=v,
          tau=tau, dt=dt, integrator=integrator, steps=1, spl=1)
-    res = np.fromfile(out, dtype=np.float64)
-    q1, v1, a1 = res[:q.size], res[q.size:q.size + v.size], res[q.size + v.size:]
+    q1, v1, a1, _ = _read_out(out, q.size, v.size)
     eq, ev, ea = step_errors(oracle, descs, q, v, tau, q1, v1, a1, dt, integrator)
     print(f"host compiled kernels, integrator {integrator}, precision {precision}: e(q') {eq.max():.3g} "
           f"e(v') {ev.max():.3g} e(qdd) {ea.max():.3g}")
