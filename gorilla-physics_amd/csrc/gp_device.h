@@ -31,12 +31,54 @@ __device__ __forceinline__ v3<R> contact_force(R z, v3<R> n, v3<R> vel, R kA, R 
   return f;
 }
 
-// sin/cos of a joint angle held in FP64: exact sincos in FP64; for FP32 the
-// angle is first reduced to [-pi, pi] in FP64.
-__device__ __forceinline__ void sincos_q(double th, double* s, double* c) { sincos(th, s, c); }
+// sin/cos of a joint angle held in FP64. Quadrant reduction th = k pi/2 + r,
+// |r| <= pi/4, with pi/2 split into three parts (the first two of 33
+// significant bits, so k C1 and k C2 are exact for |k| < 2^20; Cody & Waite
+// 1980), then the Taylor polynomials of sin and cos in r^2 (truncated after
+// r^17 / r^18: below 6e-20 relative on |r| <= pi/4). About 30 operations.
+// Angles beyond 2^19 quarter turns first lose the nearest multiple n of 2 pi,
+// held as a head and a tail (2 pi - head - tail = -6e-33): n head is split
+// into its rounded value and its exact error, th - round(n head) is exact
+// (Sterbenz), so the remainder is off by a few 1e-16 for any |th| < 1e15; an
+// infinite or NaN angle gives NaN. The library's sincos would hold its large-
+// argument reduction's registers live in every caller.
+__device__ __forceinline__ void sincos_q(double th, double* s, double* c) {
+  double k = rint(th * 0.6366197723675814);  // th * 2 / pi
+  if (!(fabs(k) < 524288.0)) {
+    const double n = rint(th * 0.15915494309189535);  // th / (2 pi)
+    const double p = n * 6.283185307179586;
+    const double e = fma(n, 6.283185307179586, -p);
+    th = fma(-n, 2.4492935982947064e-16, (th - p) - e);
+    k = rint(th * 0.6366197723675814);
+  }
+  const double r = fma(-k, 2.0222662487959506e-21, fma(-k, 6.077100506303966e-11, fma(-k, 1.5707963267341256, th)));
+  const double x2 = r * r;
+  double sn = fma(x2, 2.8114572543455206e-15, -7.647163731819816e-13);
+  sn = fma(x2, sn, 1.6059043836821613e-10);
+  sn = fma(x2, sn, -2.505210838544172e-08);
+  sn = fma(x2, sn, 2.7557319223985893e-06);
+  sn = fma(x2, sn, -0.0001984126984126984);
+  sn = fma(x2, sn, 0.008333333333333333);
+  sn = fma(x2, sn, -0.16666666666666666);
+  sn = fma(x2 * r, sn, r);
+  double cs = fma(x2, -1.5619206968586225e-16, 4.779477332387385e-14);
+  cs = fma(x2, cs, -1.1470745597729725e-11);
+  cs = fma(x2, cs, 2.08767569878681e-09);
+  cs = fma(x2, cs, -2.755731922398589e-07);
+  cs = fma(x2, cs, 2.48015873015873e-05);
+  cs = fma(x2, cs, -0.001388888888888889);
+  cs = fma(x2, cs, 0.041666666666666664);
+  cs = fma(x2, cs, -0.5);
+  cs = fma(x2, cs, 1.0);
+  const int q = int(k) & 3;  // quadrant: (s, c) -> (c, -s) per quarter turn
+  const double s1 = (q & 1) ? cs : sn;
+  const double c1 = (q & 1) ? sn : cs;
+  *s = (q & 2) ? -s1 : s1;
+  *c = ((q + 1) & 2) ? -c1 : c1;
+}
 // FP32 path: quadrant reduction th = k pi/2 + r, |r| <= pi/4, in FP64 (two
-// FMAs with pi/2 split into a head and a tail, Cody & Waite 1980), then the
-// Taylor polynomials of sin and cos in FP32. Truncation after x^9 / x^10 is
+// FMAs with pi/2 split into a head and a tail), then the Taylor polynomials of
+// sin and cos in FP32. Truncation after x^9 / x^10 is
 // below 2e-9 on |r| <= pi/4, under FP32 rounding.
 __device__ __forceinline__ void sincos_q(double th, float* s, float* c) {
   const double pio2_hi = 1.5707963267948966192313216916398;  // fl(pi / 2)

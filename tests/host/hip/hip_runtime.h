@@ -156,6 +156,8 @@ template <class T> inline T gp_host_no_shuffle() {
 }
 template <class T> inline T __shfl_xor(T, int, int = 64) { return gp_host_no_shuffle<T>(); }
 template <class T> inline T __shfl(T, int, int = 64) { return gp_host_no_shuffle<T>(); }
+inline unsigned long long __ballot(int) { return gp_host_no_shuffle<unsigned long long>(); }
+inline int __popc(unsigned x) { return __builtin_popcount(x); }
 
 // Dynamic LDS of the running launch: one guarded allocation of the launch's
 // size, shared by its workgroups in turn (the kernels that use it keep
