@@ -186,10 +186,12 @@ int launch_terms(const BucketArgs& a, double* out, void* stream);
 // kinetic and gravitational energy per lane (ke[lane], pe[lane])
 int launch_energy(const BucketArgs& a, double* ke, double* pe, void* stream);
 size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp);
-// velocity stepping of a lone floating body, `lanes` (1, 4, 8) lanes per
+// velocity stepping of a lone floating body, `lanes` (1 or 8) lanes per
 // instance sharing its `candidates` contact candidates (contact points x
-// half-spaces, plus the sphere's half-spaces), everything in registers; no
-// scratch. hipErrorInvalidValue when the candidates do not fit (more than 16).
+// half-spaces, plus the sphere's half-spaces), the working set in registers
+// and LDS. hipErrorInvalidValue when the candidates do not fit (more than
+// kVsBodyMaxCandidates).
+constexpr int kVsBodyMaxCandidates = 8;
 int launch_vs_body(const BucketArgs& a, double dt, int ccd, int n_steps, int lanes, int candidates, void* stream);
 // rows x B values between a bucket's SoA rows and a flat instance-major array
 // (lane l's values start at flat + flat_off[l]); to_soa = 1: flat -> SoA.

@@ -551,12 +551,13 @@ static int ensure_group(gp_batch* bt, int integrator) {
 }
 
 // Contact candidates of a bucket that the single-body velocity-stepping
-// kernel can take (one floating body, at most 16 point / sphere x half-space
-// pairs), else -1.
+// kernel can take (one floating body, at most kVsBodyMaxCandidates point /
+// sphere x half-space pairs), else -1: such a bucket runs the runtime-
+// topology kernel.
 static int vs_body_candidate_count(const Bucket& b) {
   if (b.nb != 1 || b.dofcls.empty() || b.dofcls[0] != 6) return -1;
   const int P = b.nc * b.nh + (b.ns > 0 ? b.nh : 0);
-  return P <= 16 ? P : -1;
+  return P <= kVsBodyMaxCandidates ? P : -1;
 }
 
 // Every bucket's launches for n_steps steps. scratch_override: where the

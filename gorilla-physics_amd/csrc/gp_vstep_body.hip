@@ -40,7 +40,7 @@ namespace gp {
 
 namespace {
 
-constexpr int kBodyCand = 8;              // candidates per instance
+constexpr int kBodyCand = kVsBodyMaxCandidates;  // candidates per instance
 constexpr int kBodyRows = 3 * kBodyCand;  // rows of the padded program
 
 template <int L> __device__ __forceinline__ double gsum(double x) {

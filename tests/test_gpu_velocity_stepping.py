@@ -114,6 +114,32 @@ def test_teacher_forced_cubes_spheres_and_quadrupeds(gp, oracle):
     assert not s.b.faults().any()
 
 
+def test_cube_in_a_corner_has_more_candidates_than_the_single_body_kernel(gp, oracle):
+    """A cube between a floor and a wall: 8 corners x 2 half-spaces = 16
+    contact candidates, more than the single-body kernel's 8, so the bucket
+    runs the runtime-topology kernel; every step still matches the oracle."""
+    from gorilla_physics_amd import builders
+    rng = np.random.default_rng(3)
+    cube = builders.build_cube(1.0, 1.0)
+    cube.add_halfspace(gp.HalfSpace.new([0.0, 0.0, 1.0], 0.0))
+    cube.add_halfspace(gp.HalfSpace.new([math.cos(0.2), 0.0, math.sin(0.2)], -0.6))
+    d = cube.model_desc()
+    descs, q, v = [], [], []
+    for _ in range(8):
+        axis = rng.normal(size=3)
+        rot = gp.UnitQuaternion.from_axis_angle(axis / np.linalg.norm(axis), rng.uniform(0, 2 * math.pi))
+        descs.append(d)
+        q += [*rot.coords(), rng.uniform(-0.1, 0.3), rng.uniform(-1, 1), rng.uniform(0.4, 0.9)]
+        v += [*rng.normal(size=3), -abs(rng.normal()), *rng.normal(size=2)]
+    for integrator in (VS, CCD):
+        s = Stepper(oracle, descs, q, v, integrator=integrator)
+        print("case cube in a corner, 16 candidates, integrator", integrator, flush=True)
+        for _ in range(30):
+            s.step(1e-2)
+        print(f"worst per-step difference {s.worst:.3g}", flush=True)
+        assert not s.b.faults().any()
+
+
 def test_particle_slide_closed_form(gp, oracle):
     """A body whose only contact point is its centre of mass, sliding at u on
     the ground: slip (u - mu g dt) / (1 + mu^2), lift-off speed mu times that."""
