@@ -322,7 +322,8 @@ def measure_config(name, w, models, steps, warmup, args, compare=True, cpu=True)
         block = block or 256
     spl = spl or min(steps, 100)
     common = dict(block_size=block, lanes_per_mech=args.lanes, lanes_per_tree=args.lanes_tree,
-                  steps_per_launch=spl, use_graph=args.graph, autotune=bool(autotune))
+                  steps_per_launch=spl, use_graph=args.graph, autotune=bool(autotune),
+                  separate_launches=int(args.separate_launches))
     f64 = run_variant(w, models, steps, warmup, precision=abi.GP_PREC_F64, kernel=abi.GP_KERNEL_AUTO, **common)
     log(name, "fast f64", f64)
     out = dict(workload=WORKLOAD_TEXT[name], instances=f64["instances"], steps=steps, warmup=warmup,
@@ -376,6 +377,8 @@ def main():
                          "options the tuner picks for them (200 steps per launch, 256-lane workgroups) so that "
                          "a profiler run of the same command sees the same launches")
     ap.add_argument("--lanes-tree", type=int, default=0)
+    ap.add_argument("--separate-launches", action="store_true",
+                    help="one launch per topology bucket instead of one grouped launch (for per-bucket kernel traces)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="leave out the other configurations' lines")

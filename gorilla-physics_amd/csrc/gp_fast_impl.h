@@ -751,7 +751,11 @@ __device__ __forceinline__ void fast_sie_body(const BucketArgs& a, int tid, doub
   bool ok = true;
   constexpr int PC = cached_bodies<Topo, R, ALLREV>();
   const FastParams<Topo, R, PC, LDS> P(pick<R>(a.bp, a.bpf), Bp, inst, 0, LDS ? (R*)GP_DYN_LDS : nullptr);
-  constexpr bool tau_reg = PC == Topo::N || FastParams<Topo, R, PC, LDS>::NT > 0;
+  // the held torques in registers for the launch, except in the FP64 chains
+  // whose per-body state fills the register file (12 joints, or 9-12 with
+  // per-lane joint kinds): they read them in each step, from L1
+  constexpr bool long64 = sizeof(R) == 8 && (Topo::N >= 12 || (!ALLREV && Topo::N >= 9));
+  constexpr bool tau_reg = (PC == Topo::N || FastParams<Topo, R, PC, LDS>::NT > 0) && !long64;
   LaunchConst<Topo, L, CONTACTS, tau_reg> lc;
   ok = load_launch_const<Topo, R, L, CONTACTS>(a, P, inst, sub, lc);
   for (int s = 0; s < my_steps; ++s)

@@ -190,9 +190,14 @@ size_t vs_scratch_doubles(int nb, int nq, int nv, int nc, int nh, int ns, int Bp
 // instance sharing its `candidates` contact candidates (contact points x
 // half-spaces, plus the sphere's half-spaces), the working set in registers
 // and LDS. hipErrorInvalidValue when the candidates do not fit (more than
-// kVsBodyMaxCandidates).
+// kVsBodyMaxCandidates). block: lanes per workgroup at 8 lanes per body (64,
+// 128 or 256, the same bits; 0 = kVsBodyBlock: one wave per workgroup, so a
+// SIMD whose wave ends takes the next one without waiting for a workgroup's
+// slowest, 1.8 % faster than 256 on the thrown cubes).
 constexpr int kVsBodyMaxCandidates = 8;
-int launch_vs_body(const BucketArgs& a, double dt, int ccd, int n_steps, int lanes, int candidates, void* stream);
+constexpr int kVsBodyBlock = 64;
+int launch_vs_body(const BucketArgs& a, double dt, int ccd, int n_steps, int lanes, int candidates, int block,
+                   void* stream);
 // rows x B values between a bucket's SoA rows and a flat instance-major array
 // (lane l's values start at flat + flat_off[l]); to_soa = 1: flat -> SoA.
 int launch_flat_xfer(double* soa, int Bp, int B, int rows, const int64_t* flat_off, double* flat, int to_soa,

@@ -596,7 +596,7 @@ static int batch_launch_all(gp_batch* bt, double dt, int integrator, int n_steps
         BucketArgs a = bucket_args(b, lanes, nullptr);
         for (int done = 0; done < n_steps; done += spl) {
           const int n = std::min(spl, n_steps - done);
-          int e = launch_vs_body(a, dt, ccd, n, lanes, cand, s);
+          int e = launch_vs_body(a, dt, ccd, n, lanes, cand, o.block_size, s);
           if (e) return fail(GP_ERR_DEVICE, "velocity-stepping launch: %s", hipGetErrorString((hipError_t)e));
           ++*launches;
         }

@@ -43,34 +43,111 @@ namespace {
 constexpr int kBodyCand = kVsBodyMaxCandidates;  // candidates per instance
 constexpr int kBodyRows = 3 * kBodyCand;  // rows of the padded program
 
+// Cross-lane moves inside an instance's 8-lane group by DPP: an 8-lane group
+// is two 4-lane banks of one 16-lane DPP row, so a quad permutation moves
+// within each bank and a row shift by 4, written to one bank of the pair
+// only (bank_mask), moves between the pair. VALU moves, where a shuffle
+// (ds_bpermute) would be a round trip through the LDS unit on each
+// dependency chain. `old` is what the lanes the bank mask leaves out keep.
+template <int CTRL, int BANKS> __device__ __forceinline__ int dpp32(int old, int x) {
+  return __builtin_amdgcn_update_dpp(old, x, CTRL, 0xF, BANKS, false);
+}
+template <int CTRL, int BANKS> __device__ __forceinline__ double dpp64(double old, double x) {
+  const long long o = __builtin_bit_cast(long long, old), v = __builtin_bit_cast(long long, x);
+  const int lo = dpp32<CTRL, BANKS>((int)o, (int)v);
+  const int hi = dpp32<CTRL, BANKS>((int)(o >> 32), (int)(v >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+constexpr int kDppRowShl4 = 0x104, kDppRowShr4 = 0x114;  // lane i takes lane i + 4 / i - 4 of its row
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E;           // quad permutations [1,0,3,2], [2,3,0,1]
+// x of the lane `o` apart (o = 1, 2, 4) in the group
+template <int O, class T> __device__ __forceinline__ T xor8(T x) {
+  if constexpr (sizeof(T) == 8) {
+    if constexpr (O == 1) return dpp64<kDppXor1, 0xF>(x, x);
+    else if constexpr (O == 2) return dpp64<kDppXor2, 0xF>(x, x);
+    else return dpp64<kDppRowShl4, 0x5>(dpp64<kDppRowShr4, 0xA>(x, x), x);
+  } else {
+    if constexpr (O == 1) return dpp32<kDppXor1, 0xF>(x, x);
+    else if constexpr (O == 2) return dpp32<kDppXor2, 0xF>(x, x);
+    else return dpp32<kDppRowShl4, 0x5>(dpp32<kDppRowShr4, 0xA>(x, x), x);
+  }
+}
+// lane S of the group: a quad broadcast of lane S % 4 in each bank, then the
+// other bank of the pair takes it
+template <int S> __device__ __forceinline__ double bcast8(double x) {
+  constexpr int q = S & 3, perm = q | (q << 2) | (q << 4) | (q << 6);
+  const double t = dpp64<perm, 0xF>(x, x);
+  if constexpr (S < 4) return dpp64<kDppRowShr4, 0xA>(t, t);
+  else return dpp64<kDppRowShl4, 0x5>(t, t);
+}
+
 template <int L> __device__ __forceinline__ double gsum(double x) {
+  if constexpr (L == 8) {
+    x += xor8<1>(x);
+    x += xor8<2>(x);
+    return x + xor8<4>(x);
+  } else {
 #pragma unroll
-  for (int o = 1; o < L; o <<= 1) x += __shfl_xor(x, o, 64);
-  return x;
+    for (int o = 1; o < L; o <<= 1) x += __shfl_xor(x, o, 64);
+    return x;
+  }
 }
 template <int L> __device__ __forceinline__ double gmax(double x) {
+  if constexpr (L == 8) {
+    x = fmax(x, xor8<1>(x));
+    x = fmax(x, xor8<2>(x));
+    return fmax(x, xor8<4>(x));
+  } else {
 #pragma unroll
-  for (int o = 1; o < L; o <<= 1) x = fmax(x, __shfl_xor(x, o, 64));
-  return x;
+    for (int o = 1; o < L; o <<= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+  }
 }
 template <int L> __device__ __forceinline__ double gmin(double x) {
+  if constexpr (L == 8) {
+    x = fmin(x, xor8<1>(x));
+    x = fmin(x, xor8<2>(x));
+    return fmin(x, xor8<4>(x));
+  } else {
 #pragma unroll
-  for (int o = 1; o < L; o <<= 1) x = fmin(x, __shfl_xor(x, o, 64));
-  return x;
+    for (int o = 1; o < L; o <<= 1) x = fmin(x, __shfl_xor(x, o, 64));
+    return x;
+  }
 }
-// lane `src` of this instance's group: its value of x
+// lane `src` of this instance's group: its value of x (src is a constant
+// wherever the loops are unrolled, and the switch folds)
 template <int L> __device__ __forceinline__ double bc(double x, int src) {
   if constexpr (L == 1) return x;
-  else return __shfl(x, src, L);
+  else if constexpr (L == 8) {
+    switch (src) {
+      case 0: return bcast8<0>(x);
+      case 1: return bcast8<1>(x);
+      case 2: return bcast8<2>(x);
+      case 3: return bcast8<3>(x);
+      case 4: return bcast8<4>(x);
+      case 5: return bcast8<5>(x);
+      case 6: return bcast8<6>(x);
+      default: return bcast8<7>(x);
+    }
+  } else return __shfl(x, src, L);
 }
 // the smallest (t, key) over the lanes, the smaller key on a tie
-template <int L> __device__ __forceinline__ void gargmin(double& t, int& key) {
-#pragma unroll
-  for (int o = 1; o < L; o <<= 1) {
-    const double t2 = __shfl_xor(t, o, 64);
-    const int k2 = __shfl_xor(key, o, 64);
-    if (t2 < t || (t2 == t && k2 < key)) { t = t2; key = k2; }
+template <int L, int O> __device__ __forceinline__ void argmin_step(double& t, int& key) {
+  double t2;
+  int k2;
+  if constexpr (L == 8) {
+    t2 = xor8<O>(t);
+    k2 = xor8<O>(key);
+  } else {
+    t2 = __shfl_xor(t, O, 64);
+    k2 = __shfl_xor(key, O, 64);
   }
+  if (t2 < t || (t2 == t && k2 < key)) { t = t2; key = k2; }
+}
+template <int L> __device__ __forceinline__ void gargmin(double& t, int& key) {
+  if constexpr (L >= 2) argmin_step<L, 1>(t, key);
+  if constexpr (L >= 4) argmin_step<L, 2>(t, key);
+  if constexpr (L >= 8) argmin_step<L, 4>(t, key);
 }
 
 __device__ __forceinline__ double dmu(int row) { return row % 3 == 0 ? kConeMu : 1.0; }
@@ -551,9 +628,8 @@ __global__ void __launch_bounds__(256) vs_body_kernel(BucketArgs a, double dt, i
 }
 
 template <int L>
-int launch_l(const BucketArgs& a, double dt, int ccd, int n_steps, hipStream_t s) {
+int launch_l(const BucketArgs& a, double dt, int ccd, int n_steps, int block, hipStream_t s) {
   const long threads = (long)a.B * L;
-  const int block = L == 1 ? 64 : 256;
   dim3 blk(block), grid((unsigned)((threads + block - 1) / block));
   const unsigned lds = (unsigned)(body_lds_pitch<L>() * sizeof(double) * block);
   hipLaunchKernelGGL((vs_body_kernel<L>), grid, blk, lds, s, a, dt, ccd, n_steps);
@@ -562,12 +638,17 @@ int launch_l(const BucketArgs& a, double dt, int ccd, int n_steps, hipStream_t s
 
 }  // namespace
 
-int launch_vs_body(const BucketArgs& a, double dt, int ccd, int n_steps, int lanes, int candidates, void* stream) {
+int launch_vs_body(const BucketArgs& a, double dt, int ccd, int n_steps, int lanes, int candidates, int block,
+                   void* stream) {
   if (a.B == 0 || n_steps <= 0) return 0;
   if (candidates > kBodyCand) return (int)hipErrorInvalidValue;
+  // each lane's LDS row is 584 bytes at 8 lanes per body: at most 256 lanes
+  // per workgroup; whole waves only
+  if (block <= 0) block = kVsBodyBlock;
+  if (block > 256 || block % 64 != 0) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
-  if (lanes == 1) return launch_l<1>(a, dt, ccd, n_steps, s);
-  return launch_l<8>(a, dt, ccd, n_steps, s);
+  if (lanes == 1) return launch_l<1>(a, dt, ccd, n_steps, 64, s);
+  return launch_l<8>(a, dt, ccd, n_steps, block, s);
 }
 
 }  // namespace gp

@@ -157,6 +157,7 @@ template <class T> inline T gp_host_no_shuffle() {
 template <class T> inline T __shfl_xor(T, int, int = 64) { return gp_host_no_shuffle<T>(); }
 template <class T> inline T __shfl(T, int, int = 64) { return gp_host_no_shuffle<T>(); }
 inline unsigned long long __ballot(int) { return gp_host_no_shuffle<unsigned long long>(); }
+#define __builtin_amdgcn_update_dpp(old, x, ctrl, row_mask, bank_mask, bound_ctrl) gp_host_no_shuffle<int>()
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
 
 // Dynamic LDS of the running launch: one guarded allocation of the launch's

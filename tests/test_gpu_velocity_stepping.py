@@ -314,6 +314,24 @@ def test_steps_in_one_launch_are_bit_identical(gp):
     assert np.array_equal(out[0][2], out[1][2]) and (out[0][3], out[1][3]) == (30, 5)
 
 
+def test_single_body_workgroup_size_changes_no_bit(gp):
+    """The single-body kernel's workgroup size (64, 128, 256 lanes: 8, 16, 32
+    cubes) changes no bit of the states, fault flags or rounds: 200 thrown
+    cubes (the last workgroup partial at every size), 30 CCD steps."""
+    from gorilla_physics_amd import batch, workloads
+    print("case single-body workgroup sizes, 200 thrown cubes", flush=True)
+    w = workloads.thrown_cubes(n=200)
+    out = []
+    for bs in (256, 128, 64):
+        b = batch.Batch([batch.Model(w["models"][0])], [200], precision=0, block_size=bs, steps_per_launch=30)
+        b.set_state(w["q"], w["v"])
+        b.step(w["dt"], CCD, 30)
+        q, v = b.get_state()
+        out.append((np.concatenate([q, v]), b.faults(), b.collision_rounds()))
+    for o in out[1:]:
+        assert all(np.array_equal(x, y) for x, y in zip(out[0], o))
+
+
 def test_velocity_stepping_rejects_mixed_plain(gp):
     from gorilla_physics_amd import batch, builders
     st = builders.build_cube(1.0, 1.0)
