@@ -157,6 +157,7 @@ __device__ __forceinline__ double dmu(int row) { return row % 3 == 0 ? kConeMu :
 struct Cand {
   int kind;  // -1 none, 0 point, 1 sphere
   int hh;    // half-space index
+  int id;    // candidate index (contact point c x half-space h: c nh + h; spheres after)
   double pb[3];
   double r;  // sphere radius
   double hpt[3], hn[3];
@@ -183,6 +184,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
     Cand& C = cd[t];
     C.kind = c < np ? 0 : (c < P ? 1 : -1);
     C.hh = C.kind == 0 ? c % nh : (C.kind == 1 ? c - np : 0);
+    C.id = c;
     C.r = C.kind == 1 ? a.shp[((size_t)SH_R * a.ns + sl) * Bp + inst] : 0.0;
     const int cc = C.kind == 0 ? c / nh : 0;
 #pragma unroll
@@ -240,13 +242,66 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
 #pragma unroll
         for (int k = 0; k < 6; ++k) vfree[k] = vc[k] + h * acc[k];
       }
+      // This round's candidates, cw: for L = 8 the group's touching ones on
+      // its first lanes in candidate order, the others after them (cd keeps
+      // the candidate order, so the layout depends on this round's contacts
+      // only, not on how the steps are split into launches). The program's
+      // active rows then come first, and the wave works on 3 x (its largest
+      // touching count) = jmax rows. A candidate that does not touch has
+      // identity rows, zero columns and a zero right-hand side: the columns
+      // past jmax would only add and subtract exact zeros, so skipping them
+      // changes no value (the sums over lanes do change order with the
+      // lanes' candidates).
+      Cand cw[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t) cw[t] = cd[t];
+      int mw = kBodyCand;
+      if constexpr (L == 8) {
+        bool touch;
+        {
+          const Cand& C = cd[0];
+          const v3<double> pw = mul(Rw, v3<double>{C.pb[0], C.pb[1], C.pb[2]}) + tw;
+          const v3<double> n{C.hn[0], C.hn[1], C.hn[2]};
+          const double dist = dot(pw - v3<double>{C.hpt[0], C.hpt[1], C.hpt[2]}, n);
+          touch = C.kind == 0 ? dist <= kInsideMargin : (C.kind == 1 ? (dist <= C.r || C.hh == forced_h) : false);
+        }
+        const unsigned long long wm = __ballot(touch);
+        const unsigned gmask = (unsigned)(wm >> (8 * ((threadIdx.x & 63) >> 3))) & 0xFFu;
+        const int na = __popc(gmask);
+        int src = 0, ca = 0, ci = 0;  // the lane whose candidate this lane takes
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          if ((gmask >> b) & 1u) {
+            if (ca == sub) src = b;
+            ++ca;
+          } else {
+            if (na + ci == sub) src = b;
+            ++ci;
+          }
+        }
+        Cand& C = cw[0];
+        C.kind = __shfl(C.kind, src, 8);
+        C.hh = __shfl(C.hh, src, 8);
+        C.id = __shfl(C.id, src, 8);
+        C.r = __shfl(C.r, src, 8);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          C.pb[k] = __shfl(C.pb[k], src, 8);
+          C.hpt[k] = __shfl(C.hpt[k], src, 8);
+          C.hn[k] = __shfl(C.hn[k], src, 8);
+        }
+        mw = 0;
+#pragma unroll
+        for (int g8 = 0; g8 < 8; ++g8) mw = max(mw, __popc((unsigned)(wm >> (8 * g8)) & 0xFFu));
+      }
+      const int jmax = 3 * mw;
       // 2. contacts
       bool act[T];
       double nact = 0.0;
       double Y[T][6][3], g[R];
 #pragma unroll
       for (int t = 0; t < T; ++t) {
-        const Cand& C = cd[t];
+        const Cand& C = cw[t];
         const v3<double> pw = mul(Rw, v3<double>{C.pb[0], C.pb[1], C.pb[2]}) + tw;
         const v3<double> n{C.hn[0], C.hn[1], C.hn[2]};
         const double dist = dot(pw - v3<double>{C.hpt[0], C.hpt[1], C.hpt[2]}, n);
@@ -303,6 +358,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           }
 #pragma unroll
           for (int c2 = 0; c2 < kBodyCand; ++c2) {
+            if (c2 >= mw) continue;  // columns past jmax are never read
             double Yc[6][3];
 #pragma unroll
             for (int k = 0; k < 6; ++k)
@@ -321,6 +377,8 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
         }
         // 5. the impulse program (the runtime-topology kernel's cone_program)
         double x[R], z[R], xb[R], rd[R], dx[R], dz[R], xi[R], lt[R], W[T][9], Wi[T][9], Lc[R][m], xall[m];
+#pragma unroll
+        for (int k = 0; k < m; ++k) xall[k] = 0.0;  // entries past jmax stay zero
         double gl = 0.0;
 #pragma unroll
         for (int t = 0; t < T; ++t)
@@ -337,7 +395,10 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
         int since_best = 0;
         auto gather = [&](const double (&loc)[R], double (&all)[m]) {
 #pragma unroll
-          for (int k = 0; k < m; ++k) all[k] = bc<L>(loc[k % R], k / R);
+          for (int k = 0; k < m; ++k) {
+            if (k >= jmax) continue;
+            all[k] = bc<L>(loc[k % R], k / R);
+          }
         };
         // Newton direction for a right-hand side xi of the scaled complementarity
         auto solve = [&]() {
@@ -354,6 +415,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           // L y = rhs: the owner of row j forms its numerator, everyone learns y_j
 #pragma unroll
           for (int j = 0; j < m; ++j) {
+            if (j >= jmax) continue;
             double s = rhs[j % R];
 #pragma unroll
             for (int k = 0; k < j; ++k) s -= Lc[j % R][k] * xall[k];
@@ -364,6 +426,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           for (int i = 0; i < R; ++i) dx[i] = 0.0;
 #pragma unroll
           for (int j = m - 1; j >= 0; --j) {
+            if (j >= jmax) continue;
             double part = 0.0;
 #pragma unroll
             for (int i = 0; i < R; ++i)
@@ -402,6 +465,10 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           return gmin<L>(alpha);
         };
         for (int it = 0; it < kConeMaxIter; ++it) {
+          // G is read from LDS in each iteration: without this compiler-only
+          // barrier the compiler keeps the rows it wrote in registers, and
+          // spills them
+          asm volatile("" ::: "memory");
           // rd = G x + g - D z
           gather(x, xall);
           double gap_l = 0.0, rmax_l = 0.0, xmax_l = 0.0;
@@ -409,7 +476,10 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           for (int i = 0; i < R; ++i) {
             double s = g[i] - dmu(i) * z[i];
 #pragma unroll
-            for (int k = 0; k < m; ++k) s += Gs[i * m + k] * xall[k];
+            for (int k = 0; k < m; ++k) {
+              if (k >= jmax) continue;
+              s += Gs[i * m + k] * xall[k];
+            }
             rd[i] = s;
             if (act[i / 3]) {
               rmax_l = fmax(rmax_l, fabs(s));
@@ -475,6 +545,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           bool pd = true;
 #pragma unroll
           for (int j = 0; j < m; ++j) {
+            if (j >= jmax) continue;  // identity rows past jmax: already their factor
             double sjj = bc<L>(Lc[j % R][j], j / R);
             double u[R];
 #pragma unroll
@@ -578,7 +649,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           const double tt = dist / -vdn;
           if (tt > 0.0 && tt < earliest) {
             earliest = tt;
-            key = sub * T + t;
+            key = C.id;
           }
         }
         gargmin<L>(earliest, key);
