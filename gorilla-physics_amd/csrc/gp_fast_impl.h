@@ -1177,7 +1177,10 @@ constexpr int max_tier_lane_bytes(std::integer_sequence<int, K...>) {
 template <class R> constexpr int grouped_tier_lane_bytes() {
   constexpr int a = max_tier_lane_bytes<R, true>(std::make_integer_sequence<int, 12>{});
   constexpr int b = max_tier_lane_bytes<R, false>(std::make_integer_sequence<int, 12>{});
-  return a > b ? a : b;
+  constexpr int m = a > b ? a : b;
+  // the tree kernel's exchange slots share the same LDS in a grouped launch
+  static_assert(m >= kTreeSlotBytes, "grouped LDS row smaller than a tree exchange slot");
+  return m;
 }
 
 // Workgroup size that fits the tier's LDS in what the device allows a

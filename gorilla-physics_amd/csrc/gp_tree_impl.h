@@ -4,8 +4,12 @@
 // sweeps level by level: every body of one depth at once, a body's parent
 // quantities (velocity, world pose, acceleration) taken from the parent's
 // lane by a cross-lane shuffle on the way out, and on the way in each parent
-// summing its children's articulated inertias and bias forces by pulling them
-// from the children's lanes (one shuffle round per child rank). Runtime
+// summing its children's articulated inertias and bias forces by reading them
+// from the children's lanes. The lanes exchange through LDS: each lane has a
+// slot of kTreeSlot doubles, written by the lane when its level is done and
+// read by its children (pass 1 and 3) or its parent (pass 2) one level later;
+// all lanes of a segment are in one wave, whose LDS instructions execute in
+// order, so no barrier is needed, only a compiler fence between levels. Runtime
 // topology: any tree of up to 64 bodies with at most kTreeMaxChildren
 // children per body, every joint type, contact points, prismatic springs;
 // semi-implicit Euler (integrators.rs:276-319), n_steps steps per launch with
@@ -20,22 +24,35 @@
 namespace gp {
 
 constexpr int kTreeMaxChildren = 8;
+// doubles per lane in the exchange slot (an articulated inertia and a force,
+// 21 + 6; odd, so a wave's 64-bit accesses spread over the banks)
+constexpr int kTreeSlot = 27;
+constexpr int kTreeSlotBytes = kTreeSlot * 8;
 
-// a spatial vector / inertia from lane `src` of this lane's L-lane segment
-template <class R, int L> __device__ __forceinline__ sv<R> shfl_sv(const sv<R>& x, int src) {
-  return {{__shfl(x.w.x, src, L), __shfl(x.w.y, src, L), __shfl(x.w.z, src, L)},
-          {__shfl(x.v.x, src, L), __shfl(x.v.y, src, L), __shfl(x.v.z, src, L)}};
+// slot <-> spatial vector / articulated inertia (FP32 values widened: exact)
+template <class R> __device__ __forceinline__ void put_sv(double* p, const sv<R>& x) {
+  p[0] = x.w.x; p[1] = x.w.y; p[2] = x.w.z; p[3] = x.v.x; p[4] = x.v.y; p[5] = x.v.z;
 }
-template <class R, int L> __device__ __forceinline__ ai6<R> shfl_ai6(const ai6<R>& I, int src) {
-  ai6<R> o;
-  o.A = s3<R>{__shfl(I.A.xx, src, L), __shfl(I.A.yy, src, L), __shfl(I.A.zz, src, L), __shfl(I.A.xy, src, L),
-              __shfl(I.A.xz, src, L), __shfl(I.A.yz, src, L)};
+template <class R> __device__ __forceinline__ sv<R> get_sv(const double* p) {
+  return {{R(p[0]), R(p[1]), R(p[2])}, {R(p[3]), R(p[4]), R(p[5])}};
+}
+template <class R> __device__ __forceinline__ void put_ai6(double* p, const ai6<R>& I) {
+  p[0] = I.A.xx; p[1] = I.A.yy; p[2] = I.A.zz; p[3] = I.A.xy; p[4] = I.A.xz; p[5] = I.A.yz;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) o.B.a[k] = __shfl(I.B.a[k], src, L);
-  o.C = s3<R>{__shfl(I.C.xx, src, L), __shfl(I.C.yy, src, L), __shfl(I.C.zz, src, L), __shfl(I.C.xy, src, L),
-              __shfl(I.C.xz, src, L), __shfl(I.C.yz, src, L)};
+  for (int k = 0; k < 9; ++k) p[6 + k] = I.B.a[k];
+  p[15] = I.C.xx; p[16] = I.C.yy; p[17] = I.C.zz; p[18] = I.C.xy; p[19] = I.C.xz; p[20] = I.C.yz;
+}
+template <class R> __device__ __forceinline__ ai6<R> get_ai6(const double* p) {
+  ai6<R> o;
+  o.A = s3<R>{R(p[0]), R(p[1]), R(p[2]), R(p[3]), R(p[4]), R(p[5])};
+#pragma unroll
+  for (int k = 0; k < 9; ++k) o.B.a[k] = R(p[6 + k]);
+  o.C = s3<R>{R(p[15]), R(p[16]), R(p[17]), R(p[18]), R(p[19]), R(p[20])};
   return o;
 }
+// a compiler-only barrier between levels: slot reads are not moved above the
+// writes of the level before (the hardware keeps one wave's LDS order)
+__device__ __forceinline__ void level_fence() { asm volatile("" ::: "memory"); }
 template <int L> __device__ __forceinline__ int seg_max(int x) {
 #pragma unroll
   for (int off = 1; off < L; off <<= 1) x = max(x, __shfl_xor(x, off, L));
@@ -70,7 +87,10 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
   const bool live = j < nb;
   // ---- topology of this lane's body
   const int par = live ? a.parent[j] : -1;
-  const int src_par = par >= 0 ? par : j;
+  // exchange slots: the segment's (its lane 0 is block lane threadIdx.x - j,
+  // blockDim being a multiple of L) and this lane's
+  double* const seg = (double*)GP_DYN_LDS + (size_t)((int)threadIdx.x - j) * kTreeSlot;
+  double* const mine = seg + j * kTreeSlot;
   int depth = -1;
   if (live) {
     depth = 0;
@@ -150,25 +170,34 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
         p_d = cvt<double>(r);
       }
     }
-    // ---- pass 1, root to leaves by depth: velocities (and world poses)
+    // ---- pass 1, root to leaves by depth: velocities (and world poses);
+    // slot: velocity [0, 6), world rotation [6, 15), world translation [15, 18)
     sv<R> vel = vj;
     m3<double> Rw = Rm_d;
     v3<double> tw = p_d;
-    for (int d = 1; d <= maxd; ++d) {
-      const sv<R> pv = shfl_sv<R, L>(vel, src_par);
-      m3<double> pR;
-      v3<double> ptw;
+    auto put_pose = [&]() {
+      put_sv(mine, vel);
       if (contacts) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) pR.a[k] = __shfl(Rw.a[k], src_par, L);
-        ptw = v3<double>{__shfl(tw.x, src_par, L), __shfl(tw.y, src_par, L), __shfl(tw.z, src_par, L)};
+        for (int k = 0; k < 9; ++k) mine[6 + k] = Rw.a[k];
+        mine[15] = tw.x; mine[16] = tw.y; mine[17] = tw.z;
       }
+    };
+    level_fence();  // the previous step's pass-3 reads come first
+    if (depth == 0) put_pose();
+    for (int d = 1; d <= maxd; ++d) {
+      level_fence();
       if (depth == d) {
-        vel = xmotion(E, r, pv) + vj;
+        const double* ps = seg + par * kTreeSlot;
+        vel = xmotion(E, r, get_sv<R>(ps)) + vj;
         if (contacts) {
-          tw = ptw + mul(pR, p_d);
+          m3<double> pR;
+#pragma unroll
+          for (int k = 0; k < 9; ++k) pR.a[k] = ps[6 + k];
+          tw = v3<double>{ps[15], ps[16], ps[17]} + mul(pR, p_d);
           Rw = mul(pR, Rm_d);
         }
+        if (d < maxd) put_pose();
       }
     }
     // velocity-product acceleration, rigid-body bias force, contact forces
@@ -229,26 +258,33 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
         }
         cI = xtix(E, r, Ia);
         cf = xtforce(E, r, pa);
-      }
-      if (d == 0) break;
-      // parents (depth d - 1) take their children's parts, one child rank at a time
-      for (int k = 0; k < maxc; ++k) {
-        const bool take = depth == d - 1 && k < nch;
-        const int src = take ? child[k] : j;
-        const ai6<R> xI = shfl_ai6<R, L>(cI, src);
-        const sv<R> xf = shfl_sv<R, L>(cf, src);
-        if (take) {
-          add_to(IA, xI);
-          pA = pA + xf;
+        if (d > 0) {  // slot: articulated inertia [0, 21), bias force [21, 27)
+          put_ai6(mine, cI);
+          put_sv(mine + 21, cf);
         }
       }
+      if (d == 0) break;
+      // parents (depth d - 1) take their children's parts, in child order
+      level_fence();
+      if (depth == d - 1) {
+#pragma unroll
+        for (int k = 0; k < kTreeMaxChildren; ++k) {
+          if (k < nch) {
+            const double* cs = seg + child[k] * kTreeSlot;
+            add_to(IA, get_ai6<R>(cs));
+            pA = pA + get_sv<R>(cs + 21);
+          }
+        }
+      }
+      level_fence();  // the slots read here are written again only in pass 3
     }
     // ---- pass 3, root to leaves by depth: accelerations, then the joint's step
     sv<R> acc{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}};  // gravity as the base's acceleration
     for (int d = 0; d <= maxd; ++d) {
-      const sv<R> pacc = shfl_sv<R, L>(acc, src_par);
+      level_fence();
       if (depth == d) {
-        sv<R> ap = xmotion(E, r, par < 0 ? sv<R>{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}} : pacc) + cb;
+        const sv<R> pacc = par < 0 ? sv<R>{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}} : get_sv<R>(seg + par * kTreeSlot);
+        sv<R> ap = xmotion(E, r, pacc) + cb;
         if (kind == K_REV || kind == K_PRI) {
           const double qdd = (u - double(dot6(U, ap))) * Dinv;
           vd[0] = qdd;
@@ -274,6 +310,7 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
           ap = ap + sv<R>{{R(x[0]), R(x[1]), R(x[2])}, {R(x[3]), R(x[4]), R(x[5])}};
         }
         acc = ap;
+        if (d < maxd) put_sv(mine, acc);  // slot: acceleration [0, 6)
       }
     }
     // ---- semi-implicit Euler of this lane's joint (integrators.rs:276-319)
@@ -316,10 +353,11 @@ template <class R>
 static int launch_wave_tree(const BucketArgs& a, double dt, int n_steps, int block, hipStream_t s) {
   const long threads = (long)a.B * a.lanes;
   dim3 blk(block), grid((unsigned)((threads + block - 1) / block));
+  const unsigned lds = (unsigned)(kTreeSlotBytes * block);
   switch (a.lanes) {
-    case 16: hipLaunchKernelGGL((wave_tree_kernel<R, 16>), grid, blk, 0, s, a, dt, n_steps); break;
-    case 32: hipLaunchKernelGGL((wave_tree_kernel<R, 32>), grid, blk, 0, s, a, dt, n_steps); break;
-    case 64: hipLaunchKernelGGL((wave_tree_kernel<R, 64>), grid, blk, 0, s, a, dt, n_steps); break;
+    case 16: hipLaunchKernelGGL((wave_tree_kernel<R, 16>), grid, blk, lds, s, a, dt, n_steps); break;
+    case 32: hipLaunchKernelGGL((wave_tree_kernel<R, 32>), grid, blk, lds, s, a, dt, n_steps); break;
+    case 64: hipLaunchKernelGGL((wave_tree_kernel<R, 64>), grid, blk, lds, s, a, dt, n_steps); break;
     default: return (int)hipErrorInvalidValue;
   }
   return (int)hipGetLastError();
