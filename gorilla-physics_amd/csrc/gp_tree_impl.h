@@ -29,25 +29,26 @@ constexpr int kTreeMaxChildren = 8;
 constexpr int kTreeSlot = 27;
 constexpr int kTreeSlotBytes = kTreeSlot * 8;
 
-// slot <-> spatial vector / articulated inertia (FP32 values widened: exact)
-template <class R> __device__ __forceinline__ void put_sv(double* p, const sv<R>& x) {
+// slot <-> spatial vector / articulated inertia, in the sweeps' precision R
+// from the slot's start (the slot is a row of doubles)
+template <class R> __device__ __forceinline__ void put_sv(R* p, const sv<R>& x) {
   p[0] = x.w.x; p[1] = x.w.y; p[2] = x.w.z; p[3] = x.v.x; p[4] = x.v.y; p[5] = x.v.z;
 }
-template <class R> __device__ __forceinline__ sv<R> get_sv(const double* p) {
-  return {{R(p[0]), R(p[1]), R(p[2])}, {R(p[3]), R(p[4]), R(p[5])}};
+template <class R> __device__ __forceinline__ sv<R> get_sv(const R* p) {
+  return {{p[0], p[1], p[2]}, {p[3], p[4], p[5]}};
 }
-template <class R> __device__ __forceinline__ void put_ai6(double* p, const ai6<R>& I) {
+template <class R> __device__ __forceinline__ void put_ai6(R* p, const ai6<R>& I) {
   p[0] = I.A.xx; p[1] = I.A.yy; p[2] = I.A.zz; p[3] = I.A.xy; p[4] = I.A.xz; p[5] = I.A.yz;
 #pragma unroll
   for (int k = 0; k < 9; ++k) p[6 + k] = I.B.a[k];
   p[15] = I.C.xx; p[16] = I.C.yy; p[17] = I.C.zz; p[18] = I.C.xy; p[19] = I.C.xz; p[20] = I.C.yz;
 }
-template <class R> __device__ __forceinline__ ai6<R> get_ai6(const double* p) {
+template <class R> __device__ __forceinline__ ai6<R> get_ai6(const R* p) {
   ai6<R> o;
-  o.A = s3<R>{R(p[0]), R(p[1]), R(p[2]), R(p[3]), R(p[4]), R(p[5])};
+  o.A = s3<R>{p[0], p[1], p[2], p[3], p[4], p[5]};
 #pragma unroll
-  for (int k = 0; k < 9; ++k) o.B.a[k] = R(p[6 + k]);
-  o.C = s3<R>{R(p[15]), R(p[16]), R(p[17]), R(p[18]), R(p[19]), R(p[20])};
+  for (int k = 0; k < 9; ++k) o.B.a[k] = p[6 + k];
+  o.C = s3<R>{p[15], p[16], p[17], p[18], p[19], p[20]};
   return o;
 }
 // a compiler-only barrier between levels: slot reads are not moved above the
@@ -91,6 +92,8 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
   // blockDim being a multiple of L) and this lane's
   double* const seg = (double*)GP_DYN_LDS + (size_t)((int)threadIdx.x - j) * kTreeSlot;
   double* const mine = seg + j * kTreeSlot;
+  R* const mineR = (R*)mine;
+  auto slotR = [&](int lane) { return (const R*)(seg + lane * kTreeSlot); };
   int depth = -1;
   if (live) {
     depth = 0;
@@ -171,12 +174,13 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
       }
     }
     // ---- pass 1, root to leaves by depth: velocities (and world poses);
-    // slot: velocity [0, 6), world rotation [6, 15), world translation [15, 18)
+    // slot: velocity (R) from the start, world rotation doubles [6, 15),
+    // world translation doubles [15, 18)
     sv<R> vel = vj;
     m3<double> Rw = Rm_d;
     v3<double> tw = p_d;
     auto put_pose = [&]() {
-      put_sv(mine, vel);
+      put_sv(mineR, vel);
       if (contacts) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) mine[6 + k] = Rw.a[k];
@@ -189,7 +193,7 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
       level_fence();
       if (depth == d) {
         const double* ps = seg + par * kTreeSlot;
-        vel = xmotion(E, r, get_sv<R>(ps)) + vj;
+        vel = xmotion(E, r, get_sv<R>(slotR(par))) + vj;
         if (contacts) {
           m3<double> pR;
 #pragma unroll
@@ -258,9 +262,9 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
         }
         cI = xtix(E, r, Ia);
         cf = xtforce(E, r, pa);
-        if (d > 0) {  // slot: articulated inertia [0, 21), bias force [21, 27)
-          put_ai6(mine, cI);
-          put_sv(mine + 21, cf);
+        if (d > 0) {  // slot (R): articulated inertia [0, 21), bias force [21, 27)
+          put_ai6(mineR, cI);
+          put_sv(mineR + 21, cf);
         }
       }
       if (d == 0) break;
@@ -270,7 +274,7 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
 #pragma unroll
         for (int k = 0; k < kTreeMaxChildren; ++k) {
           if (k < nch) {
-            const double* cs = seg + child[k] * kTreeSlot;
+            const R* cs = slotR(child[k]);
             add_to(IA, get_ai6<R>(cs));
             pA = pA + get_sv<R>(cs + 21);
           }
@@ -283,7 +287,7 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
     for (int d = 0; d <= maxd; ++d) {
       level_fence();
       if (depth == d) {
-        const sv<R> pacc = par < 0 ? sv<R>{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}} : get_sv<R>(seg + par * kTreeSlot);
+        const sv<R> pacc = par < 0 ? sv<R>{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}} : get_sv<R>(slotR(par));
         sv<R> ap = xmotion(E, r, pacc) + cb;
         if (kind == K_REV || kind == K_PRI) {
           const double qdd = (u - double(dot6(U, ap))) * Dinv;
@@ -310,7 +314,7 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
           ap = ap + sv<R>{{R(x[0]), R(x[1]), R(x[2])}, {R(x[3]), R(x[4]), R(x[5])}};
         }
         acc = ap;
-        if (d < maxd) put_sv(mine, acc);  // slot: acceleration [0, 6)
+        if (d < maxd) put_sv(mineR, acc);  // slot (R): acceleration from the start
       }
     }
     // ---- semi-implicit Euler of this lane's joint (integrators.rs:276-319)
