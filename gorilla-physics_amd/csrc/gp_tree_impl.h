@@ -93,7 +93,6 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
   double* const seg = (double*)GP_DYN_LDS + (size_t)((int)threadIdx.x - j) * kTreeSlot;
   double* const mine = seg + j * kTreeSlot;
   R* const mineR = (R*)mine;
-  auto slotR = [&](int lane) { return (const R*)(seg + lane * kTreeSlot); };
   int depth = -1;
   if (live) {
     depth = 0;
@@ -193,7 +192,7 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
       level_fence();
       if (depth == d) {
         const double* ps = seg + par * kTreeSlot;
-        vel = xmotion(E, r, get_sv<R>(slotR(par))) + vj;
+        vel = xmotion(E, r, get_sv<R>((const R*)ps)) + vj;
         if (contacts) {
           m3<double> pR;
 #pragma unroll
@@ -274,7 +273,7 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
 #pragma unroll
         for (int k = 0; k < kTreeMaxChildren; ++k) {
           if (k < nch) {
-            const R* cs = slotR(child[k]);
+            const R* cs = (const R*)(seg + child[k] * kTreeSlot);
             add_to(IA, get_ai6<R>(cs));
             pA = pA + get_sv<R>(cs + 21);
           }
@@ -287,7 +286,7 @@ __device__ __forceinline__ void wave_tree_body(const BucketArgs& a, int tid, dou
     for (int d = 0; d <= maxd; ++d) {
       level_fence();
       if (depth == d) {
-        const sv<R> pacc = par < 0 ? sv<R>{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}} : get_sv<R>(slotR(par));
+        const sv<R> pacc = par < 0 ? sv<R>{{R(0), R(0), R(0)}, {R(0), R(0), R(a.g)}} : get_sv<R>((const R*)(seg + par * kTreeSlot));
         sv<R> ap = xmotion(E, r, pacc) + cb;
         if (kind == K_REV || kind == K_PRI) {
           const double qdd = (u - double(dot6(U, ap))) * Dinv;

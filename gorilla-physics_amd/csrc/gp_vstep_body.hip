@@ -42,6 +42,9 @@ namespace {
 
 constexpr int kBodyCand = kVsBodyMaxCandidates;  // candidates per instance
 constexpr int kBodyRows = 3 * kBodyCand;  // rows of the padded program
+// doubles of an instance's parked state: q, v (the step's start), the
+// current q and v, v_free, the world rotation and translation
+constexpr int kStash = 7 + 6 + 7 + 6 + 6 + 9 + 3;
 
 // Cross-lane moves inside an instance's 8-lane group by DPP: an 8-lane group
 // is two 4-lane banks of one 16-lane DPP row, so a quad permutation moves
@@ -227,8 +230,8 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
         ok = chol6(M6, LM) && ok;
       }
       // 1. free velocity, no contact forces: vdot = M^-1 (tau - v x* M v) - X a_g
-      const m3<double> Rw = mul(Rt, quat_mat(qc[0], qc[1], qc[2], qc[3]));
-      const v3<double> tw = pt + mul(Rt, v3<double>{qc[4], qc[5], qc[6]});
+      m3<double> Rw = mul(Rt, quat_mat(qc[0], qc[1], qc[2], qc[3]));
+      v3<double> tw = pt + mul(Rt, v3<double>{qc[4], qc[5], qc[6]});
       double vfree[6];
       {
         const sv<double> V{{vc[0], vc[1], vc[2]}, {vc[3], vc[4], vc[5]}};
@@ -375,6 +378,25 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
               }
           }
         }
+        // The instance's state that outlives the interior point waits in LDS
+        // meanwhile, its registers free for the program: every lane of the
+        // instance holds the same bits, its first lane writes, all read back.
+        double* const stash = lds + (size_t)blockDim.x * pitch + (size_t)(threadIdx.x / L) * kStash;
+        auto park = [&](bool back) {
+          auto mv = [&](double& x, int k) {
+            if (back) x = stash[k];
+            else if (sub == 0) stash[k] = x;
+          };
+#pragma unroll
+          for (int k = 0; k < 7; ++k) { mv(q[k], k); mv(qc[k], 7 + k); }
+#pragma unroll
+          for (int k = 0; k < 6; ++k) { mv(v[k], 14 + k); mv(vc[k], 20 + k); mv(vfree[k], 26 + k); }
+#pragma unroll
+          for (int k = 0; k < 9; ++k) mv(Rw.a[k], 32 + k);
+          mv(tw.x, 41); mv(tw.y, 42); mv(tw.z, 43);
+          asm volatile("" ::: "memory");
+        };
+        park(false);
         // 5. the impulse program (the runtime-topology kernel's cone_program)
         double x[R], z[R], xb[R], rd[R], dx[R], dz[R], xi[R], lt[R], W[T][9], Wi[T][9], Lc[R][m], xall[m];
 #pragma unroll
@@ -617,6 +639,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           if (act[i / 3] && !isfinite(x[i])) bad = 1.0;
         }
         ok = ok && gmax<L>(bad) == 0.0;
+        park(true);
         // 6. v_next = v_free + Y x
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
@@ -702,7 +725,7 @@ template <int L>
 int launch_l(const BucketArgs& a, double dt, int ccd, int n_steps, int block, hipStream_t s) {
   const long threads = (long)a.B * L;
   dim3 blk(block), grid((unsigned)((threads + block - 1) / block));
-  const unsigned lds = (unsigned)(body_lds_pitch<L>() * sizeof(double) * block);
+  const unsigned lds = (unsigned)((body_lds_pitch<L>() * block + block / L * kStash) * sizeof(double));
   hipLaunchKernelGGL((vs_body_kernel<L>), grid, blk, lds, s, a, dt, ccd, n_steps);
   return (int)hipGetLastError();
 }
@@ -718,7 +741,9 @@ int launch_vs_body(const BucketArgs& a, double dt, int ccd, int n_steps, int lan
   if (block <= 0) block = kVsBodyBlock;
   if (block > 256 || block % 64 != 0) return (int)hipErrorInvalidValue;
   hipStream_t s = (hipStream_t)stream;
-  if (lanes == 1) return launch_l<1>(a, dt, ccd, n_steps, 64, s);
+  // one lane per body: its LDS row holds all 24 rows of G (4.6 KB), so 16
+  // lanes per workgroup (74 KB)
+  if (lanes == 1) return launch_l<1>(a, dt, ccd, n_steps, 16, s);
   return launch_l<8>(a, dt, ccd, n_steps, block, s);
 }
 

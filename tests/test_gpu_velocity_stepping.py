@@ -35,7 +35,7 @@ class Stepper:
     compared with the oracle from the same state: the new state, and the
     rounds of the collision loop."""
 
-    def __init__(self, oracle, descs, q, v, tau=None, integrator=VS):
+    def __init__(self, oracle, descs, q, v, tau=None, integrator=VS, **opts):
         from gorilla_physics_amd import batch
         self.oracle, self.descs, self.integrator = oracle, descs, integrator
         models, self._m = [], {}
@@ -43,7 +43,7 @@ class Stepper:
             if id(d) not in self._m:
                 self._m[id(d)] = batch.Model(d)
             models.append(self._m[id(d)])
-        self.b = batch.Batch(models, [1] * len(models), precision=0)
+        self.b = batch.Batch(models, [1] * len(models), precision=0, **opts)
         self.q, self.v = np.array(q, float), np.array(v, float)
         self.tau = np.zeros(self.b.nv_total) if tau is None else np.array(tau, float)
         self.b.set_state(self.q, self.v)
@@ -75,9 +75,11 @@ class Stepper:
         return qn, vn
 
 
-def test_teacher_forced_cubes_spheres_and_quadrupeds(gp, oracle):
+@pytest.mark.parametrize("lanes", [0, 1])
+def test_teacher_forced_cubes_spheres_and_quadrupeds(gp, oracle, lanes):
     """Random cubes and spheres near a tilted ground (touching, sliding,
-    resting, in flight) and quadrupeds standing on it, in one batch."""
+    resting, in flight) and quadrupeds standing on it, in one batch; the lone
+    bodies with 8 lanes each (the default) and with one."""
     from gorilla_physics_amd import builders
     rng = np.random.default_rng(0)
     cube = builders.build_cube(1.0, 1.0)
@@ -106,8 +108,8 @@ def test_teacher_forced_cubes_spheres_and_quadrupeds(gp, oracle):
             qq[7:] = rng.normal(size=dq.nq - 7) * 0.2
             q += list(qq)
             v += list(rng.normal(size=dq.nv) * 0.3)
-    s = Stepper(oracle, descs, q, v)
-    print("case velocity stepping teacher-forced mix of 48", flush=True)
+    s = Stepper(oracle, descs, q, v, lanes_per_mech=lanes)
+    print(f"case velocity stepping teacher-forced mix of 48, lanes_per_mech {lanes}", flush=True)
     for _ in range(40):
         s.step(1e-2)
     print(f"worst per-step difference {s.worst:.3g}", flush=True)
