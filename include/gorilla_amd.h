@@ -174,7 +174,10 @@ typedef struct gp_batch_opts {
   int32_t kernel;            /* GP_KERNEL_*                                   */
   int32_t steps_per_launch;  /* steps fused in one launch by every kernel
                                 (0 = 100)                                     */
-  int32_t block_size;        /* threads per workgroup (0 = 256)               */
+  int32_t block_size;        /* threads per workgroup, 64/128/256 (0 = 256;
+                                grouped launches 128; the single-body
+                                velocity-stepping kernel 64, or 16 at one
+                                lane per body)                                */
   int32_t lanes_per_mech;    /* FAST free bodies with contacts: lanes sharing
                                 one instance's contact points, 1/2/4/8 (0 = 4);
                                 velocity stepping of a lone floating body with
