@@ -22,8 +22,12 @@
 // lane owning row j broadcasts it, every lane updates its own rows; the
 // triangular solves broadcast each solved entry. Sums over rows of different
 // lanes are xor butterflies (commutative adds: every lane holds the same
-// bits and takes the same branches). M is the body's spatial inertia in its
-// frame (the joint-space mass matrix of a floating joint).
+// bits and takes the same branches). At 8 lanes, each collision round puts
+// the touching candidates on an instance's first lanes and the wave skips the
+// program's columns past 3 x its largest touching count; broadcasts and sums
+// are DPP moves; the instance's long-lived state waits in LDS while the
+// interior point runs. M is the body's spatial inertia in its frame (the
+// joint-space mass matrix of a floating joint).
 #include <hip/hip_runtime.h>
 
 #include "gp_cone.h"
