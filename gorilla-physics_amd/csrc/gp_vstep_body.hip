@@ -402,7 +402,11 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
         };
         park(false);
         // 5. the impulse program (the runtime-topology kernel's cone_program)
-        double x[R], z[R], xb[R], rd[R], dx[R], dz[R], xi[R], lt[R], W[T][9], Wi[T][9], Lc[R][m], xall[m];
+        // dinv: 1 / (this lane's rows' factor diagonal), one division per
+        // column of the factorisation and none in the solves
+        double x[R], z[R], xb[R], rd[R], dx[R], dz[R], xi[R], lt[R], W[T][9], Wi[T][9], Lc[R][m], xall[m], dinv[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) dinv[i] = 1.0;
 #pragma unroll
         for (int k = 0; k < m; ++k) xall[k] = 0.0;  // entries past jmax stay zero
         double gl = 0.0;
@@ -445,7 +449,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
             double s = rhs[j % R];
 #pragma unroll
             for (int k = 0; k < j; ++k) s -= Lc[j % R][k] * xall[k];
-            xall[j] = bc<L>(s / Lc[j % R][j], j / R);  // only the owner's quotient is kept
+            xall[j] = bc<L>(s * dinv[j % R], j / R);  // only the owner's product is kept
           }
           // L^T dx = y: column j of L is spread over the rows' owners
 #pragma unroll
@@ -458,7 +462,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
             for (int i = 0; i < R; ++i)
               if (sub * R + i > j) part += Lc[i][j] * dx[i];
             const double num = xall[j] - gsum<L>(part);
-            xall[j] = bc<L>(num / Lc[j % R][j], j / R);
+            xall[j] = bc<L>(num * dinv[j % R], j / R);
             if (j / R == sub) dx[j % R] = xall[j];
           }
 #pragma unroll
@@ -584,12 +588,16 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
               for (int i = 0; i < R; ++i) u[i] -= Lc[i][k] * ljk;
             }
             pd = pd && sjj > 0.0;
-            const double ljj = sqrt(sjj);
+            const double ljj = sqrt(sjj), rj = 1.0 / ljj;
 #pragma unroll
             for (int i = 0; i < R; ++i) {
               const int gi = sub * R + i;
-              if (gi == j) Lc[i][j] = ljj;
-              else if (gi > j) Lc[i][j] = u[i] / ljj;
+              if (gi == j) {
+                Lc[i][j] = ljj;
+                dinv[i] = rj;
+              } else if (gi > j) {
+                Lc[i][j] = u[i] * rj;
+              }
             }
           }
           if (!pd) break;
