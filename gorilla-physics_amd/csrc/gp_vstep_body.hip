@@ -535,7 +535,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
           for (int t = 0; t < T; ++t) {
             if (act[t]) {
               const double sc[3] = {dmu(0) * x[3 * t], x[3 * t + 1], x[3 * t + 2]};
-              nt_scaling(sc, &z[3 * t], W[t], Wi[t]);
+              nt_scaling<true>(sc, &z[3 * t], W[t], Wi[t]);
             } else {
 #pragma unroll
               for (int k = 0; k < 9; ++k) { W[t][k] = 0.0; Wi[t][k] = 0.0; }
