@@ -10,6 +10,35 @@
 
 namespace gp {
 
+// Cross-lane moves within aligned groups of up to 8 lanes by DPP (VALU moves;
+// a shuffle, ds_bpermute, would be a round trip through the LDS unit): an
+// 8-lane group is two 4-lane banks of one 16-lane DPP row, so a quad
+// permutation moves within each bank and a row shift by 4, written to one
+// bank of the pair only (bank_mask), moves between the pair. `old` is what
+// the lanes the bank mask leaves out keep.
+constexpr int kDppRowShl4 = 0x104, kDppRowShr4 = 0x114;  // lane i takes lane i + 4 / i - 4 of its row
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E;           // quad permutations [1,0,3,2], [2,3,0,1]
+template <int CTRL, int BANKS> __device__ __forceinline__ int dpp32(int old, int x) {
+  return __builtin_amdgcn_update_dpp(old, x, CTRL, 0xF, BANKS, false);
+}
+template <int CTRL, int BANKS, class T> __device__ __forceinline__ T dpp_move(T old, T x) {
+  if constexpr (sizeof(T) == 8) {
+    const long long o = __builtin_bit_cast(long long, old), v = __builtin_bit_cast(long long, x);
+    const int lo = dpp32<CTRL, BANKS>((int)o, (int)v);
+    const int hi = dpp32<CTRL, BANKS>((int)(o >> 32), (int)(v >> 32));
+    return __builtin_bit_cast(T, ((long long)hi << 32) | (long long)(unsigned)lo);
+  } else {
+    return __builtin_bit_cast(T, dpp32<CTRL, BANKS>(__builtin_bit_cast(int, old), __builtin_bit_cast(int, x)));
+  }
+}
+// x of the lane O apart (O = 1, 2, 4) in its aligned group
+template <int O, class T> __device__ __forceinline__ T lane_xor(T x) {
+  static_assert(O == 1 || O == 2 || O == 4, "DPP lane xor within 8 lanes");
+  if constexpr (O == 1) return dpp_move<kDppXor1, 0xF>(x, x);
+  else if constexpr (O == 2) return dpp_move<kDppXor2, 0xF>(x, x);
+  else return dpp_move<kDppRowShl4, 0x5>(dpp_move<kDppRowShr4, 0xA>(x, x), x);
+}
+
 // Hunt-Crossley normal force + regularised Coulomb friction, force from the
 // half-space on the point (contact.rs:260-302). z^1.5 is z * sqrt(z): NaN for
 // z < 0 exactly like powf, and fmax(NaN, 0) = 0 like Rust's f64::max.

@@ -50,34 +50,10 @@ constexpr int kBodyRows = 3 * kBodyCand;  // rows of the padded program
 // current q and v, v_free, the world rotation and translation
 constexpr int kStash = 7 + 6 + 7 + 6 + 6 + 9 + 3;
 
-// Cross-lane moves inside an instance's 8-lane group by DPP: an 8-lane group
-// is two 4-lane banks of one 16-lane DPP row, so a quad permutation moves
-// within each bank and a row shift by 4, written to one bank of the pair
-// only (bank_mask), moves between the pair. VALU moves, where a shuffle
-// (ds_bpermute) would be a round trip through the LDS unit on each
-// dependency chain. `old` is what the lanes the bank mask leaves out keep.
-template <int CTRL, int BANKS> __device__ __forceinline__ int dpp32(int old, int x) {
-  return __builtin_amdgcn_update_dpp(old, x, CTRL, 0xF, BANKS, false);
-}
+// x of the lane `O` apart (O = 1, 2, 4) in the instance's 8-lane group
+template <int O, class T> __device__ __forceinline__ T xor8(T x) { return lane_xor<O>(x); }
 template <int CTRL, int BANKS> __device__ __forceinline__ double dpp64(double old, double x) {
-  const long long o = __builtin_bit_cast(long long, old), v = __builtin_bit_cast(long long, x);
-  const int lo = dpp32<CTRL, BANKS>((int)o, (int)v);
-  const int hi = dpp32<CTRL, BANKS>((int)(o >> 32), (int)(v >> 32));
-  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
-}
-constexpr int kDppRowShl4 = 0x104, kDppRowShr4 = 0x114;  // lane i takes lane i + 4 / i - 4 of its row
-constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E;           // quad permutations [1,0,3,2], [2,3,0,1]
-// x of the lane `o` apart (o = 1, 2, 4) in the group
-template <int O, class T> __device__ __forceinline__ T xor8(T x) {
-  if constexpr (sizeof(T) == 8) {
-    if constexpr (O == 1) return dpp64<kDppXor1, 0xF>(x, x);
-    else if constexpr (O == 2) return dpp64<kDppXor2, 0xF>(x, x);
-    else return dpp64<kDppRowShl4, 0x5>(dpp64<kDppRowShr4, 0xA>(x, x), x);
-  } else {
-    if constexpr (O == 1) return dpp32<kDppXor1, 0xF>(x, x);
-    else if constexpr (O == 2) return dpp32<kDppXor2, 0xF>(x, x);
-    else return dpp32<kDppRowShl4, 0x5>(dpp32<kDppRowShr4, 0xA>(x, x), x);
-  }
+  return dpp_move<CTRL, BANKS>(old, x);
 }
 // lane S of the group: a quad broadcast of lane S % 4 in each bank, then the
 // other bank of the pair takes it
