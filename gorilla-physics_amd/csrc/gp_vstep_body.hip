@@ -184,6 +184,7 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
 #pragma unroll
   for (int k = 0; k < 6; ++k) v[k] = a.v[(size_t)k * Bp + inst];
 
+  int last_rounds = 0;
   for (int step = 0; step < n_steps; ++step) {
     double qc[7], vc[6], vn[6];
 #pragma unroll
@@ -678,11 +679,13 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
     }
     ok = ok && done;  // the collision loop gave up: a fault, the state still advances
     if (!done) advance_floating(qc, vn, t_remain, qc);
+    // the state, vdot and the rounds leave the registers once, after the
+    // launch's last step (a fault flag when it happens)
     bool fin = true;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
       const double vd = (vn[k] - v[k]) / dt;
-      if (sub == 0) a.vdot[(size_t)k * Bp + inst] = vd;
+      if (sub == 0 && step + 1 == n_steps) a.vdot[(size_t)k * Bp + inst] = vd;
       v[k] = vn[k];
       fin = fin && isfinite(v[k]);
     }
@@ -691,14 +694,15 @@ __device__ void vs_body(const BucketArgs& a, int tid, double dt, int ccd, int n_
       q[k] = qc[k];
       fin = fin && isfinite(q[k]);
     }
-    if (sub == 0) {
+    if (sub == 0 && !(ok && fin)) a.fault[inst] = 1;
+    last_rounds = rounds;
+  }
+  if (sub == 0) {
 #pragma unroll
-      for (int k = 0; k < 7; ++k) a.q[(size_t)k * Bp + inst] = q[k];
+    for (int k = 0; k < 7; ++k) a.q[(size_t)k * Bp + inst] = q[k];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) a.v[(size_t)k * Bp + inst] = v[k];
-      if (!(ok && fin)) a.fault[inst] = 1;
-      a.rounds[inst] = rounds;
-    }
+    for (int k = 0; k < 6; ++k) a.v[(size_t)k * Bp + inst] = v[k];
+    a.rounds[inst] = last_rounds;
   }
 }
 
