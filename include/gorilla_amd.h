@@ -184,6 +184,7 @@ typedef struct gp_batch_opts {
                                 at most 8 contact candidates: 1 = one lane
                                 per body, else 8 lanes per body             */
   int32_t use_graph;         /* capture multi-launch rollouts in a hipGraph   */
+                             /* (a chunk of one kernel is launched directly)  */
   int32_t separate_launches; /* 1: one launch per topology bucket instead of
                                 one grouped launch over all FAST buckets      */
   int32_t lanes_per_tree;    /* FAST floating base with 4 three-joint legs:
@@ -262,7 +263,9 @@ int gp_step(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, voi
 /* Do now the one-time work that gp_step(batch, dt, integrator, n_steps) would
  * do on its first call: the grouped launch's table, bucket scratch and, with
  * use_graph, the hipGraphs it replays (one per chunk of steps_per_launch steps
- * and one for a shorter last chunk), so that later gp_step calls only launch.
+ * and one for a shorter last chunk; a chunk whose graph holds a single kernel
+ * is launched directly, which costs less than the graph's launch), so that
+ * later gp_step calls only launch.
  * Nothing is stepped. No reference counterpart (setup of the batched launch). */
 int gp_step_prepare(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, void* stream);
 /* gp_step without any allocation, for callers that cannot allocate device

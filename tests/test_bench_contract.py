@@ -60,7 +60,7 @@ def test_timed_call_replays_prepared_graphs(gp):
                  "--no-compare")
     r = out["roofline"]
     assert r["launches"] == 1 and r["steps_per_launch"] == 20
-    # wall-clock per step includes one graph launch and a synchronisation, not an instantiation
+    # wall-clock per step includes one launch and a synchronisation, not a capture or an instantiation
     assert r["launch_ms"] <= out["ms_per_step"] * 20
 
 

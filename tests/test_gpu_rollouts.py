@@ -178,24 +178,30 @@ def test_simulate_loop_matches_oracle(gp, oracle):
 
 
 def test_graph_replays_and_grouped_launch_are_exact(gp):
-    """A hipGraph-captured rollout replayed several times (with eager work in
-    between) and a grouped launch over all buckets give bit-identical states
-    to plain per-bucket launches."""
+    """A hipGraph-captured rollout of per-bucket launches replayed several
+    times (with eager work in between), and a grouped launch over all buckets
+    (one kernel per chunk: launched directly, without its graph), give
+    bit-identical states to plain per-bucket launches."""
     from gorilla_physics_amd import batch, workloads
     w = workloads.mixed(n=2048)
     print("case graph/grouped", flush=True)
     ref, _ = batch.batch_from_workload(w, precision=1, separate_launches=1, use_graph=0)
-    g, _ = batch.batch_from_workload(w, precision=1, use_graph=1)
+    g, _ = batch.batch_from_workload(w, precision=1, separate_launches=1, use_graph=1)
+    gr, _ = batch.batch_from_workload(w, precision=1, use_graph=1)
     assert g.n_buckets > 2
     for rep in range(3):
         ref.step(1e-3, 0, 250)
         g.step(1e-3, 0, 250)
+        gr.step(1e-3, 0, 250)
         g.dynamics()  # eager work on the same batch between graph replays
         qa, va = ref.get_state()
-        qb, vb = g.get_state()
-        # instances that diverge (non-finite) must do so in both
-        assert np.array_equal(qa, qb, equal_nan=True) and np.array_equal(va, vb, equal_nan=True), rep
+        for other in (g, gr):
+            qb, vb = other.get_state()
+            # instances that diverge (non-finite) must do so in both
+            assert np.array_equal(qa, qb, equal_nan=True) and np.array_equal(va, vb, equal_nan=True), rep
     ms, launches = g.last_timing()
+    assert launches == 3 * g.n_buckets and ms > 0
+    ms, launches = gr.last_timing()
     assert launches == 3 and ms > 0
 
 
