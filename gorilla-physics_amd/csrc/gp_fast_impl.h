@@ -200,27 +200,27 @@ template <class Topo, class R, int CN> struct LdsTier {
 
 // Unit quaternion (w, x, y, z) of a rotation matrix (row-major), by
 // Shepperd's method: the largest of 1 + trace and 1 + 2 R_kk - trace under
-// the square root.
+// the square root. The case is picked by selects, not branches (the same
+// arithmetic in each case): a launch's prologue converts every cached body's
+// rotation, and branches there keep the compiler from issuing the next body's
+// parameter loads before this one's conversion is done.
 template <class T>
 __device__ __forceinline__ void rot_to_quat(const m3<T>& M, T* o) {
   const T m00 = M.a[0], m01 = M.a[1], m02 = M.a[2], m10 = M.a[3], m11 = M.a[4], m12 = M.a[5], m20 = M.a[6],
           m21 = M.a[7], m22 = M.a[8];
   const T tr = m00 + m11 + m22;
-  T w, x, y, z;
-  if (tr >= m00 && tr >= m11 && tr >= m22) {
-    const T r = sqrt(T(1) + tr), h = T(0.5) / r;
-    w = T(0.5) * r; x = (m21 - m12) * h; y = (m02 - m20) * h; z = (m10 - m01) * h;
-  } else if (m00 >= m11 && m00 >= m22) {
-    const T r = sqrt(T(1) + m00 - m11 - m22), h = T(0.5) / r;
-    x = T(0.5) * r; w = (m21 - m12) * h; y = (m01 + m10) * h; z = (m02 + m20) * h;
-  } else if (m11 >= m22) {
-    const T r = sqrt(T(1) + m11 - m00 - m22), h = T(0.5) / r;
-    y = T(0.5) * r; w = (m02 - m20) * h; x = (m01 + m10) * h; z = (m12 + m21) * h;
-  } else {
-    const T r = sqrt(T(1) + m22 - m00 - m11), h = T(0.5) / r;
-    z = T(0.5) * r; w = (m10 - m01) * h; x = (m02 + m20) * h; y = (m12 + m21) * h;
-  }
-  o[0] = w; o[1] = x; o[2] = y; o[3] = z;
+  const bool c0 = tr >= m00 && tr >= m11 && tr >= m22;     // w largest
+  const bool c1 = !c0 && m00 >= m11 && m00 >= m22;         // x largest
+  const bool c2 = !c0 && !c1 && m11 >= m22;                // y largest (else z)
+  const T s0 = T(1) + tr, s1 = T(1) + m00 - m11 - m22, s2 = T(1) + m11 - m00 - m22, s3 = T(1) + m22 - m00 - m11;
+  const T r = sqrt(c0 ? s0 : c1 ? s1 : c2 ? s2 : s3), h = T(0.5) / r;
+  const T big = T(0.5) * r;
+  const T a = (m21 - m12) * h, b = (m02 - m20) * h, c = (m10 - m01) * h;
+  const T d = (m01 + m10) * h, e = (m02 + m20) * h, f = (m12 + m21) * h;
+  o[0] = c0 ? big : c1 ? a : c2 ? b : c;
+  o[1] = c0 ? a : c1 ? big : c2 ? d : e;
+  o[2] = c0 ? b : c1 ? d : c2 ? big : f;
+  o[3] = c0 ? c : c1 ? e : c2 ? f : big;
 }
 
 template <class T>
