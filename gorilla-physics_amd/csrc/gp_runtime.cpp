@@ -891,6 +891,7 @@ static int transfer(gp_batch* bt, const double* hq, const double* hv, double* oq
 static int transfer_dev(gp_batch* bt, const double* q, const double* v, double* oq, double* ov, int which,
                         void* stream) {
   hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
+  if (stream) bt->foreign_stream = true;
   const bool to_dev = oq == nullptr && ov == nullptr;
   for (Bucket& b : bt->buckets) {
     int e = 0;
@@ -1071,6 +1072,7 @@ static int step_prepare(gp_batch* bt, double dt, int integrator, int n_steps, hi
 int gp_step_prepare(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* stream) {
   int st = check_step_args(bt, dt, integrator, n_steps);
   if (st) return st;
+  if (stream) bt->foreign_stream = true;
   return step_prepare(bt, dt, integrator, n_steps, stream ? (hipStream_t)stream : bt->stream);
 }
 
@@ -1078,6 +1080,7 @@ int gp_step(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps, void* 
   int st = check_step_args(bt, dt, integrator, n_steps);
   if (st) return st;
   hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
+  if (stream) bt->foreign_stream = true;
   if ((st = step_prepare(bt, dt, integrator, n_steps, s))) return st;
   int64_t launches = 0;
   // the timing events bracket the launches only (any preparation is done above)
@@ -1124,6 +1127,7 @@ int gp_step_noalloc(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps
   if ((st = noalloc_scratch(&scr))) return st;
   int64_t launches = 0;
   hipStream_t s = stream ? (hipStream_t)stream : bt->stream;
+  if (stream) bt->foreign_stream = true;
   if ((st = batch_launch_all(bt, dt, integrator, n_steps, s, &launches, scr))) return st;
   bt->timing_valid = false;
   return GP_OK;
@@ -1132,7 +1136,11 @@ int gp_step_noalloc(gp_batch* bt, double dt, int32_t integrator, int32_t n_steps
 int gp_batch_sync(gp_batch* bt) {
   if (!bt) return fail(GP_ERR_INVALID, "null batch");
   HIP_TRY(hipStreamSynchronize(bt->stream));
-  HIP_TRY(hipDeviceSynchronize());
+  if (bt->foreign_stream) {
+    // work the caller put on streams of its own: wait for the whole device
+    HIP_TRY(hipDeviceSynchronize());
+    bt->foreign_stream = false;
+  }
   return GP_OK;
 }
 

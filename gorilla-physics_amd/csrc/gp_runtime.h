@@ -118,6 +118,9 @@ struct gp_batch {
   std::vector<int> inst_bucket, inst_lane;
   gp_batch_opts opts{};
   hipStream_t stream = nullptr;
+  // a caller's own stream has carried this batch's work since the last
+  // gp_batch_sync (which then waits for the whole device)
+  bool foreign_stream = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timing_valid = false;
   int64_t last_launches = 0;

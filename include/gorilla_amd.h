@@ -281,7 +281,9 @@ int gp_step_prepare(gp_batch* batch, double dt, int32_t integrator, int32_t n_st
  * race on it: serialise them. Not timed (gp_batch_last_timing refers to the
  * last gp_step). */
 int gp_step_noalloc(gp_batch* batch, double dt, int32_t integrator, int32_t n_steps, void* stream);
-/* Block until the batch's work on its stream is done. */
+/* Block until the batch's work on its stream is done; when a call since the
+ * last gp_batch_sync was given a stream of the caller's own, until all work
+ * on the device is done. */
 int gp_batch_sync(gp_batch* batch);
 
 /* Joint accelerations of the last step's first dynamics evaluation (the vdot
