@@ -205,6 +205,37 @@ def test_graph_replays_and_grouped_launch_are_exact(gp):
     assert launches == 3 and ms > 0
 
 
+def test_step_on_a_callers_stream(gp):
+    """gp_step on a stream the caller created, not the batch's own: after
+    gp_batch_sync the states equal, bit for bit, those of the same rollout
+    on the batch's stream (the sync then waits for the whole device). The
+    stream is non-blocking, so the host copies that read the state do not
+    wait for it by themselves."""
+    import ctypes as C
+    from gorilla_physics_amd import batch, workloads
+    w = workloads.arms(n=4096)
+    print("case caller stream", flush=True)
+    hip = _hip()
+    hip.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    st = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(st), 1) == 0  # hipStreamNonBlocking
+    try:
+        ref, _ = batch.batch_from_workload(w, precision=1)
+        b, _ = batch.batch_from_workload(w, precision=1)
+        for rep in range(2):
+            ref.step(w["dt"], 0, 200)
+            b.step(w["dt"], 0, 200, stream=st.value)
+            ref.sync()
+            b.sync()
+            qa, va = ref.get_state()
+            qb, vb = b.get_state()
+            assert np.array_equal(qa, qb, equal_nan=True) and np.array_equal(va, vb, equal_nan=True), rep
+        del b, ref
+    finally:
+        hip.hipStreamDestroy(st)
+
+
 def _hip():
     """The HIP runtime the library itself links (the loader hands back the
     instance already mapped into the process)."""
